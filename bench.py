@@ -1,0 +1,218 @@
+"""Benchmark of the BVH query path on one MI355X.
+
+Headline (the metric's configuration): dense unsigned distance over a 256^3 grid on
+[-1, 1]^3 against the perturbed level-8 icosphere (1,310,720 faces).  A step is one
+cuBVH.unsigned_distance call over all 16,777,216 grid points, inputs resident in HBM.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` = M queries/s over the timed steps (wall clock
+bracketed by device synchronizations).  `roofline` prices the dominant kernel at the
+algorithmic bytes of the reference's own BVH walk on the same workload
+(tests/golden/algo_bytes.json) divided by its average duration measured with HIP
+events on the launch stream.  `cpu_baseline` times the CPU restatement of the
+reference's BVH walk (oracle/) on a seeded sample of the same grid on host cores.
+--extras adds fwd+bwd, watertight SDF, ray tracing, raystab and floodfill figures.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _algo_bytes():
+    p = ROOT / "tests" / "golden" / "algo_bytes.json"
+    if not p.exists():
+        return None
+    return json.loads(p.read_text())
+
+
+def _time_kernel(fn, steps, warmup):
+    """Wall time (synchronized) and HIP-event time per call on the current stream."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        fn()
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return wall, float(np.mean(ev)) / 1e3
+
+
+def cpu_baseline(v, f, pts, sample=1 << 17, threads=None):
+    """Reference BVH walk restated on the CPU (oracle), on a seeded sample of the grid."""
+    from oracle import oracle as O
+
+    threads = threads or min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    R = O.RefBvh(v, f)
+    build_s = time.perf_counter() - t0
+    idx = np.random.default_rng(123).choice(len(pts), sample, replace=False)
+    sub = np.ascontiguousarray(pts[idx])
+    R.udf(sub[:4096], return_uvw=False, nthreads=threads)  # warm caches / thread pool
+    t0 = time.perf_counter()
+    R.udf(sub, return_uvw=False, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": sample / dt / 1e6,
+        "unit": "Mqueries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample} of the 256^3 grid points (seeded), reference BVH4 build ({build_s:.2f} s, 1 thread) "
+                  f"and closest_triangle walk restated in C++ (oracle/cubvh_oracle.cpp), {threads} OpenMP threads",
+    }
+
+
+def run_extras(dev, steps, warmup, out):
+    import torch
+
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    ab = _algo_bytes() or {"workloads": {}}
+    # differentiable UDF: jittered level-6 sphere, 2^20 N(0, 0.4^2) points, loss = mean(udf^2)
+    v6, f6 = S.jittered_sphere(6)
+    bvh6 = cubvh_amd.cuBVH(v6, f6)
+    P = torch.from_numpy(S.normal_points(1 << 20, 0.4, seed=0)).to(dev)
+    V6 = torch.from_numpy(v6).to(dev)
+
+    def fwd():
+        bvh6.unsigned_distance(P, return_uvw=True)
+
+    def fwdbwd():
+        p = P.detach().requires_grad_(True)
+        v = V6.detach().requires_grad_(True)
+        d, _, _ = cubvh_amd.udf(bvh6, p, v)
+        (d * d).mean().backward()
+
+    _, tf = _time_kernel(fwd, steps * 4, warmup)
+    _, tfb = _time_kernel(fwdbwd, steps * 4, warmup)
+    out["fwd_bwd_level6"] = {"queries": 1 << 20, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3,
+                             "ratio": tfb / tf, "fwd_bwd_Mq_s": (1 << 20) / tfb / 1e6}
+
+    # watertight SDF on the headline workload
+    v8, f8, _ = S.perturbed_sphere(8, seed=0)
+    bvh8 = cubvh_amd.cuBVH(v8, f8)
+    G = torch.from_numpy(S.grid_points(256)).to(dev)
+    _, t = _time_kernel(lambda: bvh8.signed_distance(G, mode="watertight"), steps, 1)
+    out["sdf_watertight_grid256"] = {"Mq_s": G.shape[0] / t / 1e6, "ms": t * 1e3}
+
+    # floodfill of the occupancy udf < 2/256
+    d, _, _ = bvh8.unsigned_distance(G)
+    occ = (d < 2.0 / 256).reshape(1, 256, 256, 256)
+    _, t = _time_kernel(lambda: cubvh_amd.floodfill(occ), steps, 1)
+    out["floodfill_256"] = {"ms": t * 1e3, "Mvoxels_s": occ.numel() / t / 1e6,
+                            "GB_s_5B_per_voxel": 5 * occ.numel() / t / 1e9}
+    del G, d, occ
+
+    # soup: camera rays (16 x 1920 x 1080) and raystab on 2^22 points
+    vs, fs = S.soup(8, seed=0)
+    t0 = time.perf_counter()
+    bvhs = cubvh_amd.cuBVH(vs, fs)
+    out["soup_build_s"] = time.perf_counter() - t0
+    ro, rd = S.camera_rays()
+    RO = torch.from_numpy(ro).to(dev)
+    RD = torch.from_numpy(rd).to(dev)
+    _, t = _time_kernel(lambda: bvhs.ray_trace(RO, RD), max(2, steps // 2), 1)
+    out["rays_soup_1080p_x16"] = {"Mrays_s": RO.shape[0] / t / 1e6, "ms": t * 1e3}
+    del RO, RD
+    Q = torch.from_numpy(S.uniform_points(1 << 22, seed=0)).to(dev)
+    _, t = _time_kernel(lambda: bvhs.signed_distance(Q, mode="raystab"), max(2, steps // 2), 1)
+    out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--extras", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--res", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
+              f"multi-GPU form in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    dev = torch.device("cuda:0")
+    v, f, meta = S.perturbed_sphere(8, seed=0)
+    t0 = time.perf_counter()
+    bvh = cubvh_amd.cuBVH(v, f)
+    build_s = time.perf_counter() - t0
+    pts = S.grid_points(args.res)
+    P = torch.from_numpy(pts).to(dev)
+    n = P.shape[0]
+    dist = torch.empty(n, dtype=torch.float32, device=dev)
+    face = torch.empty(n, dtype=torch.int64, device=dev)
+    lib = cubvh_amd._lib.load()
+    import ctypes
+
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+    def step():
+        cubvh_amd._lib.check(lib.cubvh_unsigned_distance(bvh._h, dev.index, n, P.data_ptr(), dist.data_ptr(),
+                                                         face.data_ptr(), None, stream))
+
+    wall, ev = _time_kernel(step, args.steps, args.warmup)
+    value = n * args.steps / wall / 1e6
+    ab = _algo_bytes()
+    roof = None
+    if ab and "udf_level8_grid256" in ab["workloads"] and args.res == 256:
+        bpq = ab["workloads"]["udf_level8_grid256"]["bytes_per_query"]
+        achieved = bpq * n / ev / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "bytes_per_query": bpq, "kernel_ms": ev * 1e3}
+    result = {
+        "metric": "M UDF queries/s @1.3M tris 256^3 (plain vs fwd+bwd vs batched), % HBM roofline",
+        "value": value,
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"dense unsigned_distance, {args.res}^3 grid on [-1,1]^3, perturbed icosphere "
+                               f"level 8 ({f.shape[0]} faces, seed 0)", "queries": n, "build_s": build_s},
+        "roofline": roof,
+    }
+    extra = {}
+    if args.extras:
+        run_extras(dev, max(2, args.steps // 2), 1, extra)
+        result["extra"] = extra
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(v, f, pts)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

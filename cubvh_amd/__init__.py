@@ -1,0 +1,34 @@
+"""cubvh_amd: the BVH query path of ashawkey/cubvh, MI355X-native (HIP, gfx950).
+
+Drop-in for the reference package's exports (cubvh/__init__.py:1-11): use
+``import cubvh_amd as cubvh``.  The BVH query path (cuBVH, floodfill) runs in
+libcubvh_amd.so; the other reference names raise NotImplementedError.
+Additions: :func:`udf` (differentiable unsigned distance).
+"""
+from .api import (
+    cuBVH,
+    HashTable,
+    cuHashTable,
+    sparse_marching_cubes,
+    sparse_marching_cubes_cpu,
+    floodfill,
+    fill_holes,
+    merge_vertices,
+    decimate,
+    parallel_decimate,
+)
+from .autograd import udf
+
+__all__ = [
+    "cuBVH",
+    "HashTable",
+    "cuHashTable",
+    "sparse_marching_cubes",
+    "sparse_marching_cubes_cpu",
+    "floodfill",
+    "fill_holes",
+    "merge_vertices",
+    "decimate",
+    "parallel_decimate",
+    "udf",
+]

@@ -1,0 +1,407 @@
+// capi.cpp -- the C ABI declared in include/cubvh_amd.h.
+//
+// Host side of the library: argument checks, the host BVH (built without touching
+// any GPU), lazy per-device upload, and kernel launches on the caller's stream.
+// Every entry point catches C++ exceptions and reports a status code instead.
+#include "cubvh_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bvh.hpp"
+#include "query.hpp"
+
+using namespace cubvh_amd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+int fail_hip(hipError_t e, const char* what) {
+    return fail(CUBVH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr int kMaxDevices = 64;
+
+// Makes `device` current for the scope and restores the caller's device.
+struct DeviceScope {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceScope(int device) {
+        err = hipGetDevice(&prev);
+        if (err == hipSuccess && prev != device) err = hipSetDevice(device);
+    }
+    ~DeviceScope() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct cubvh_bvh {
+    HostBvh host;
+    std::mutex mu;
+    struct DevCopy {
+        WideNode* nodes = nullptr;
+        float4* tris = nullptr;
+    };
+    DevCopy dev[kMaxDevices];
+
+    ~cubvh_bvh() {
+        for (int d = 0; d < kMaxDevices; ++d) {
+            if (!dev[d].nodes && !dev[d].tris) continue;
+            DeviceScope g(d);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(dev[d].nodes);
+            (void)hipFree(dev[d].tris);
+        }
+    }
+
+    // Uploads on first use (the reference's lazy upload, src/api_gpu.cu:76-78).
+    int ensure_device(int device, hipStream_t stream) {
+        if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+        std::lock_guard<std::mutex> lock(mu);
+        DevCopy& c = dev[device];
+        if (c.nodes) return CUBVH_OK;
+        DeviceScope g(device);
+        if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+        const size_t nb = host.nodes.size() * sizeof(WideNode);
+        const size_t tb = host.tris.size() * sizeof(float);
+        WideNode* n = nullptr;
+        float4* t = nullptr;
+        hipError_t e = hipMalloc(&n, nb);
+        if (e != hipSuccess) return fail_hip(e, "hipMalloc(nodes)");
+        e = hipMalloc(&t, tb);
+        if (e != hipSuccess) {
+            (void)hipFree(n);
+            return fail_hip(e, "hipMalloc(triangles)");
+        }
+        e = hipMemcpyAsync(n, host.nodes.data(), nb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(t, host.tris.data(), tb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(n);
+            (void)hipFree(t);
+            return fail_hip(e, "upload");
+        }
+        c.nodes = n;
+        c.tris = t;
+        return CUBVH_OK;
+    }
+
+    QueryArgs args(int device, int64_t n, void* stream) const {
+        QueryArgs q;
+        q.n = n;
+        q.nodes = dev[device].nodes;
+        q.tris = dev[device].tris;
+        q.face0_slot = host.face0_slot;
+        q.stack_need = host.stack_need;
+        q.stream = stream;
+        return q;
+    }
+};
+
+#define CUBVH_GUARD_BEGIN try {
+#define CUBVH_GUARD_END                                                        \
+    }                                                                          \
+    catch (const std::bad_alloc&) {                                            \
+        return fail(CUBVH_ERR_NOMEM, "host allocation failed");               \
+    }                                                                          \
+    catch (const std::exception& ex) {                                         \
+        return fail(CUBVH_ERR_ARG, std::string("unexpected error: ") + ex.what()); \
+    }
+
+namespace {
+
+int finish_bvh(cubvh_bvh* b, cubvh_bvh** out) {
+    if (stack_capacity_for(b->host.stack_need) < 0) {
+        delete b;
+        return fail(CUBVH_ERR_STATE, "tree too deep for the traversal stack");
+    }
+    *out = b;
+    g_last_error.clear();
+    return CUBVH_OK;
+}
+
+// Leaf-order 12-float internal triangle record -> 48-byte exported record.
+void export_record(const float* in, int32_t* out) {
+    float rec[9] = {in[0], in[1], in[2], in[4], in[5], in[6], in[8], in[9], in[10]};
+    int32_t id32;
+    std::memcpy(&id32, &in[3], 4);
+    int64_t id = id32;
+    std::memcpy(out, rec, 36);
+    out[9] = 0;
+    std::memcpy(out + 10, &id, 8);
+}
+
+// kmode: 0 unsigned, 1 watertight, 2 raystab
+int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* positions, float* distances,
+                 int64_t* face_id, float* uvw, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh) return fail(CUBVH_ERR_ARG, "bvh is null");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    if (n == 0) return CUBVH_OK;
+    if (!positions || !distances || !face_id) return fail(CUBVH_ERR_ARG, "null query buffer");
+    if (n > ((int64_t)1 << 37)) return fail(CUBVH_ERR_ARG, "too many queries (at most 2^37 per call)");
+    int rc = bvh->ensure_device(device, (hipStream_t)stream);
+    if (rc != CUBVH_OK) return rc;
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    QueryArgs q = bvh->args(device, n, stream);
+    q.in0 = positions;
+    q.out_f = distances;
+    q.out_id = face_id;
+    q.out_v = uvw;
+    hipError_t e = launch_distance(q, kmode);
+    if (e != hipSuccess) return fail_hip(e, "distance kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cubvh_last_error(void) { return g_last_error.c_str(); }
+
+int cubvh_abi_version(void) { return (1 << 16) | 0; }
+
+int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
+                     int32_t leaf_size, cubvh_bvh** out) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!vertices || !faces) return fail(CUBVH_ERR_ARG, "vertices and faces must not be null");
+    if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "mesh must have vertices and faces");
+    if (n_faces > (int64_t)0x7fffffff) return fail(CUBVH_ERR_ARG, "too many faces");
+    if (leaf_size == 0) leaf_size = 4;
+    std::vector<float> tri9(9 * (size_t)n_faces);
+    std::vector<int64_t> ids((size_t)n_faces);
+    for (int64_t f = 0; f < n_faces; ++f) {
+        for (int k = 0; k < 3; ++k) {
+            int32_t v = faces[3 * f + k];
+            if (v < 0 || v >= n_vertices)
+                return fail(CUBVH_ERR_ARG, "face " + std::to_string(f) + " references vertex " + std::to_string(v) +
+                                               " outside [0, " + std::to_string(n_vertices) + ")");
+            std::memcpy(&tri9[9 * f + 3 * k], vertices + 3 * (int64_t)v, 12);
+        }
+        ids[f] = f;
+    }
+    cubvh_bvh* b = new cubvh_bvh;
+    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host);
+    if (!err.empty()) {
+        delete b;
+        return fail(CUBVH_ERR_ARG, err);
+    }
+    return finish_bvh(b, out);
+    CUBVH_GUARD_END
+}
+
+void cubvh_bvh_destroy(cubvh_bvh* bvh) { delete bvh; }
+
+int cubvh_bvh_info(const cubvh_bvh* bvh, int64_t info[5]) {
+    if (!bvh || !info) return fail(CUBVH_ERR_ARG, "null argument");
+    info[0] = (int64_t)bvh->host.nodes.size();
+    info[1] = bvh->host.n_faces;
+    info[2] = bvh->host.depth;
+    info[3] = bvh->host.stack_need;
+    info[4] = bvh->host.leaf_size;
+    return CUBVH_OK;
+}
+
+int cubvh_bvh_export(const cubvh_bvh* bvh, int32_t* nodes, int32_t* triangles) {
+    if (!bvh || !nodes || !triangles) return fail(CUBVH_ERR_ARG, "null argument");
+    std::memcpy(nodes, bvh->host.nodes.data(), bvh->host.nodes.size() * sizeof(WideNode));
+    for (int64_t s = 0; s < bvh->host.n_faces; ++s) export_record(&bvh->host.tris[12 * s], triangles + 12 * s);
+    return CUBVH_OK;
+}
+
+int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* triangles, int64_t n_faces,
+                     cubvh_bvh** out) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!nodes || !triangles || n_nodes <= 0 || n_faces <= 0)
+        return fail(CUBVH_ERR_STATE, "state has no nodes or no triangles");
+    cubvh_bvh* b = new cubvh_bvh;
+    HostBvh& h = b->host;
+    h.nodes.resize((size_t)n_nodes);
+    std::memcpy(h.nodes.data(), nodes, (size_t)n_nodes * sizeof(WideNode));
+    h.n_faces = n_faces;
+    h.tris.assign(12 * (size_t)n_faces, 0.0f);
+    int64_t leaf_max = 1;
+    for (int64_t s = 0; s < n_faces; ++s) {
+        const int32_t* r = triangles + 12 * s;
+        float* o = &h.tris[12 * s];
+        float rec[9];
+        std::memcpy(rec, r, 36);
+        int64_t id;
+        std::memcpy(&id, r + 10, 8);
+        if (id < 0 || id >= n_faces) {
+            delete b;
+            return fail(CUBVH_ERR_STATE, "triangle record has an out-of-range face id");
+        }
+        int32_t id32 = (int32_t)id;
+        o[0] = rec[0]; o[1] = rec[1]; o[2] = rec[2];
+        std::memcpy(&o[3], &id32, 4);
+        o[4] = rec[3]; o[5] = rec[4]; o[6] = rec[5];
+        o[8] = rec[6]; o[9] = rec[7]; o[10] = rec[8];
+    }
+    for (const WideNode& w : h.nodes)
+        for (int k = 0; k < 4; ++k)
+            if (w.child[k] != CHILD_EMPTY && w.child[k] < 0) leaf_max = std::max<int64_t>(leaf_max, (w.child[k] & 7) + 1);
+    h.leaf_size = (int32_t)leaf_max;
+    std::string err = validate_bvh(h);
+    if (!err.empty()) {
+        delete b;
+        return fail(CUBVH_ERR_STATE, "invalid state: " + err);
+    }
+    return finish_bvh(b, out);
+    CUBVH_GUARD_END
+}
+
+int cubvh_bvh_import_reference(const int32_t* nodes9, int64_t n_nodes, const int32_t* triangles12,
+                               int64_t n_faces, int32_t leaf_size, cubvh_bvh** out) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!nodes9 || !triangles12 || n_nodes <= 0 || n_faces <= 0)
+        return fail(CUBVH_ERR_STATE, "state has no nodes or no triangles");
+    // Structural check of the reference nodes (bvh.cuh:14-21): internal nodes point to
+    // a run of children inside the array, leaves to a triangle range.
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        int32_t l = nodes9[9 * i + 6], r = nodes9[9 * i + 7];
+        if (l < 0) {
+            int64_t b = -(int64_t)l - 1, e = -(int64_t)r - 1;
+            if (b < 0 || e < b || e > n_faces) return fail(CUBVH_ERR_STATE, "reference leaf range out of bounds");
+        } else if (l >= r || r > n_nodes || l == 0) {
+            return fail(CUBVH_ERR_STATE, "reference child range out of bounds");
+        }
+    }
+    if (leaf_size == 0) leaf_size = 4;
+    std::vector<float> tri9(9 * (size_t)n_faces);
+    std::vector<int64_t> ids((size_t)n_faces);
+    std::vector<uint8_t> seen((size_t)n_faces, 0);
+    for (int64_t s = 0; s < n_faces; ++s) {
+        const int32_t* r = triangles12 + 12 * s;
+        std::memcpy(&tri9[9 * s], r, 36);
+        int64_t id;
+        std::memcpy(&id, r + 10, 8);
+        if (id < 0 || id >= n_faces || seen[id])
+            return fail(CUBVH_ERR_STATE, "reference triangle ids are not a permutation of 0..M-1");
+        seen[id] = 1;
+        ids[s] = id;
+    }
+    cubvh_bvh* b = new cubvh_bvh;
+    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host);
+    if (!err.empty()) {
+        delete b;
+        return fail(CUBVH_ERR_STATE, err);
+    }
+    return finish_bvh(b, out);
+    CUBVH_GUARD_END
+}
+
+int cubvh_bvh_triangles(const cubvh_bvh* bvh, float* tri_vertices) {
+    if (!bvh || !tri_vertices) return fail(CUBVH_ERR_ARG, "null argument");
+    const HostBvh& h = bvh->host;
+    for (int64_t s = 0; s < h.n_faces; ++s) {
+        const float* in = &h.tris[12 * s];
+        int32_t id;
+        std::memcpy(&id, &in[3], 4);
+        float* o = tri_vertices + 9 * (int64_t)id;
+        o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
+        o[3] = in[4]; o[4] = in[5]; o[5] = in[6];
+        o[6] = in[8]; o[7] = in[9]; o[8] = in[10];
+    }
+    return CUBVH_OK;
+}
+
+int cubvh_bvh_upload(cubvh_bvh* bvh, int device, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh) return fail(CUBVH_ERR_ARG, "bvh is null");
+    return bvh->ensure_device(device, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* positions, float* distances,
+                            int64_t* face_id, float* uvw, void* stream) {
+    return run_distance(bvh, device, n, 0, positions, distances, face_id, uvw, stream);
+}
+
+int cubvh_signed_distance(cubvh_bvh* bvh, int device, int64_t n, uint32_t mode, const float* positions,
+                          float* distances, int64_t* face_id, float* uvw, void* stream) {
+    // src/bvh.cu:440-465: mode 0 is watertight, every other value raystab
+    return run_distance(bvh, device, n, mode == 0 ? 1 : 2, positions, distances, face_id, uvw, stream);
+}
+
+int cubvh_ray_trace(cubvh_bvh* bvh, int device, int64_t n, const float* rays_o, const float* rays_d,
+                    float* positions, int64_t* face_id, float* depth, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh) return fail(CUBVH_ERR_ARG, "bvh is null");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative ray count");
+    if (n == 0) return CUBVH_OK;
+    if (!rays_o || !rays_d || !positions || !face_id || !depth) return fail(CUBVH_ERR_ARG, "null ray buffer");
+    if (n > ((int64_t)1 << 37)) return fail(CUBVH_ERR_ARG, "too many rays (at most 2^37 per call)");
+    int rc = bvh->ensure_device(device, (hipStream_t)stream);
+    if (rc != CUBVH_OK) return rc;
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    QueryArgs q = bvh->args(device, n, stream);
+    q.in0 = rays_o;
+    q.in1 = rays_d;
+    q.out_v = positions;
+    q.out_id = face_id;
+    q.out_f = depth;
+    hipError_t e = launch_ray(q);
+    if (e != hipSuccess) return fail_hip(e, "ray kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_floodfill(int device, const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D, int32_t* labels,
+                    void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (B < 0 || H < 0 || W < 0 || D < 0) return fail(CUBVH_ERR_ARG, "negative grid dimension");
+    const int64_t ntot = (int64_t)B * H * W * D;
+    if (ntot == 0) return CUBVH_OK;
+    if (ntot >= ((int64_t)1 << 31)) return fail(CUBVH_ERR_ARG, "grid must have fewer than 2^31 voxels");
+    if (!grid || !labels) return fail(CUBVH_ERR_ARG, "null grid buffer");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    hipError_t e = launch_floodfill(grid, B, H, W, D, labels, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "floodfill kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,
+                       const int64_t* face_id, const float* uvw, const float* vertices, int64_t n_vertices,
+                       const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
+                       void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    if (n == 0) return CUBVH_OK;
+    if (!grad_dist || !positions || !face_id || !uvw || !vertices || !faces)
+        return fail(CUBVH_ERR_ARG, "null input buffer");
+    if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "empty mesh");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    hipError_t e = launch_udf_backward(n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
+                                       grad_points, grad_vertices, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "backward kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+}  // extern "C"

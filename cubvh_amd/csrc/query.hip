@@ -1,0 +1,466 @@
+// query.hip -- BVH traversal kernels for gfx950 (wave64).
+//
+// One lane per query.  Each lane walks the 4-wide tree nearest-child-first: the
+// four child bounds of a node arrive as six 16-byte loads (structure-of-arrays
+// inside the 128-byte node), the four box tests run as independent lane-local
+// arithmetic, the surviving children are ordered with a 5-comparator network, the
+// nearest one is descended into directly and the others are pushed on a per-lane
+// stack that lives in LDS (entry = child reference + its box key, laid out
+// [depth][lane] so that a wave's pushes and pops hit distinct banks).  Popped
+// entries are re-checked against the current bound, so work found stale after a
+// better hit is skipped (the reference prunes only at push, src/bvh.cu:333-337).
+//
+// Parity rules (independent of the tree shape, see build.cpp for the conservative
+// boxes):
+//   distance: smallest fp32 squared distance, ties to the lowest original face id
+//   ray:      smallest hit parameter t < 1000, ties to the lowest original face id
+//   any-hit:  exists a face with t < 1000
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh.hpp"
+#include "geom.hpp"
+#include "query.hpp"
+
+namespace cubvh_amd {
+namespace dev {
+
+constexpr float MAX_DIST = 1000.0f;                  // src/bvh.cu:17
+constexpr float MAX_DIST_SQ = MAX_DIST * MAX_DIST;   // src/bvh.cu:18
+constexpr float NORMAL_EPS = 1e-6f;                  // src/bvh.cu:360
+// Children whose key is within this relative factor of the current best are still
+// visited, which absorbs fp32 rounding differences between box and triangle tests.
+constexpr float BOUND_SLACK = 1.0f + 1.0f / 4096.0f;
+
+struct Tree {
+    const WideNode* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    int64_t face0_slot;
+};
+
+__device__ __forceinline__ void load_tri(const float4* __restrict__ tris, int32_t slot, f3& a, f3& b, f3& c,
+                                         int32_t& id) {
+    float4 t0 = tris[3 * slot + 0];
+    float4 t1 = tris[3 * slot + 1];
+    float4 t2 = tris[3 * slot + 2];
+    a = xyz(t0);
+    b = xyz(t1);
+    c = xyz(t2);
+    id = __float_as_int(t0.w);
+}
+
+__device__ __forceinline__ void leaf_range(int32_t ref, int32_t& first, int32_t& count) {
+    first = (int32_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
+    count = (ref & 7) + 1;
+}
+
+// Ascending 4-element sort network on (key, ref).
+__device__ __forceinline__ void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
+    if (kb < ka) {
+        float tk = ka; ka = kb; kb = tk;
+        int32_t tr = ra; ra = rb; rb = tr;
+    }
+}
+__device__ __forceinline__ void sort4(float k[4], int32_t r[4]) {
+    cswap(k[0], r[0], k[2], r[2]);
+    cswap(k[1], r[1], k[3], r[3]);
+    cswap(k[0], r[0], k[1], r[1]);
+    cswap(k[2], r[2], k[3], r[3]);
+    cswap(k[1], r[1], k[2], r[2]);
+}
+
+// Per-lane traversal stack.  The first LDS_N entries live in LDS, laid out
+// [entry][lane] (entry e of lane t at e * BLOCK + t) so that a wave's accesses at
+// equal depth hit distinct banks; deeper entries, which are rare, spill to a
+// private array (scratch).  STACK_TOTAL bounds the depth of the whole stack: a
+// nearest-first walk of a 4-wide tree of depth D holds at most 3 * D + 1 entries,
+// and the library refuses trees that would need more.
+constexpr int STACK_TOTAL = 64;
+
+template <int BLOCK, int LDS_N>
+struct LaneStack {
+    int32_t* ref;
+    float* key;
+    int tid;
+    int sp;
+    int32_t spill_ref[STACK_TOTAL - LDS_N];
+    float spill_key[STACK_TOTAL - LDS_N];
+    __device__ __forceinline__ void push(int32_t r, float k) {
+        if (sp < LDS_N) {
+            ref[sp * BLOCK + tid] = r;
+            key[sp * BLOCK + tid] = k;
+        } else if (sp < STACK_TOTAL) {
+            spill_ref[sp - LDS_N] = r;
+            spill_key[sp - LDS_N] = k;
+        } else {
+            return;  // unreachable for accepted trees
+        }
+        ++sp;
+    }
+    // top entry after a decrement of sp by the caller
+    __device__ __forceinline__ float key_at(int i) const {
+        return i < LDS_N ? key[i * BLOCK + tid] : spill_key[i - LDS_N];
+    }
+    __device__ __forceinline__ int32_t ref_at(int i) const {
+        return i < LDS_N ? ref[i * BLOCK + tid] : spill_ref[i - LDS_N];
+    }
+};
+
+__device__ __forceinline__ void box_dist2_4(const WideNode& nd, f3 p, float d[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float dx = fmaxf(fmaxf(nd.lo_x[k] - p.x, p.x - nd.hi_x[k]), 0.0f);
+        float dy = fmaxf(fmaxf(nd.lo_y[k] - p.y, p.y - nd.hi_y[k]), 0.0f);
+        float dz = fmaxf(fmaxf(nd.lo_z[k] - p.z, p.z - nd.hi_z[k]), 0.0f);
+        d[k] = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    }
+}
+
+__device__ __forceinline__ WideNode load_node(const WideNode* __restrict__ nodes, int32_t i) {
+    const float4* src = reinterpret_cast<const float4*>(nodes + i);
+    WideNode nd;
+    float4* dst = reinterpret_cast<float4*>(&nd);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = src[k];
+    return nd;
+}
+
+// Nearest face to p.  Returns the leaf slot (-1 if nothing within MAX_DIST) and its
+// squared distance in best_d2.
+template <int BLOCK, int LDS_N>
+__device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st, float& best_d2, int32_t& best_id) {
+    float best = MAX_DIST_SQ;
+    float bound = MAX_DIST_SQ * BOUND_SLACK;
+    int32_t bslot = -1, bid = 0x7fffffff;
+    int32_t ref = 0;
+    st.sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            WideNode nd = load_node(T.nodes, ref);
+            float k[4];
+            int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
+            box_dist2_4(nd, p, k);
+            sort4(k, r);
+            if (k[0] <= bound) {
+                if (k[3] <= bound) st.push(r[3], k[3]);
+                if (k[2] <= bound) st.push(r[2], k[2]);
+                if (k[1] <= bound) st.push(r[1], k[1]);
+                ref = r[0];
+                continue;
+            }
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, c;
+                int32_t id;
+                load_tri(T.tris, first + j, a, b, c, id);
+                float d2 = tri_dist2(a, b, c, p);
+                if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
+                    best = d2;
+                    bslot = first + j;
+                    bid = id;
+                    bound = best * BOUND_SLACK;
+                }
+            }
+        }
+        // pop the next entry that can still hold a closer face
+        ref = CHILD_EMPTY;
+        while (st.sp > 0) {
+            --st.sp;
+            if (st.key_at(st.sp) <= bound) {
+                ref = st.ref_at(st.sp);
+                break;
+            }
+        }
+        if (ref == CHILD_EMPTY) break;
+    }
+    best_d2 = best;
+    best_id = bid;
+    return bslot;
+}
+
+// Entry/exit parameters of the ray against the 4 child boxes; key = entry when the
+// box is hit within [0, tmax_limit], +inf otherwise.
+__device__ __forceinline__ void box_ray_4(const WideNode& nd, f3 o, f3 inv, float tlimit, float k[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float tx0 = (nd.lo_x[j] - o.x) * inv.x, tx1 = (nd.hi_x[j] - o.x) * inv.x;
+        float ty0 = (nd.lo_y[j] - o.y) * inv.y, ty1 = (nd.hi_y[j] - o.y) * inv.y;
+        float tz0 = (nd.lo_z[j] - o.z) * inv.z, tz1 = (nd.hi_z[j] - o.z) * inv.z;
+        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        bool hit = nd.child[j] != CHILD_EMPTY && tn <= tf && tf >= 0.0f && tn <= tlimit;
+        k[j] = hit ? tn : INFINITY;
+    }
+}
+
+__device__ __forceinline__ f3 ray_inverse(f3 d) {
+    // Exact reciprocal; zero components are replaced by a tiny value of the same
+    // sign so that no slab product becomes 0 * inf.
+    float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+    float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+    float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+    return mk3(1.0f / x, 1.0f / y, 1.0f / z);
+}
+
+// Closest hit.  Returns the leaf slot (-1 on a miss) and t in best_t.
+template <int BLOCK, int LDS_N>
+__device__ int32_t closest_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st, float& best_t) {
+    f3 inv = ray_inverse(d);
+    float best = MAX_DIST;
+    float bound = MAX_DIST * BOUND_SLACK;
+    int32_t bslot = -1, bid = 0x7fffffff;
+    int32_t ref = 0;
+    st.sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            WideNode nd = load_node(T.nodes, ref);
+            float k[4];
+            int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
+            box_ray_4(nd, o, inv, bound, k);
+            sort4(k, r);
+            if (k[0] <= bound) {
+                if (k[3] <= bound) st.push(r[3], k[3]);
+                if (k[2] <= bound) st.push(r[2], k[2]);
+                if (k[1] <= bound) st.push(r[1], k[1]);
+                ref = r[0];
+                continue;
+            }
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, c;
+                int32_t id;
+                load_tri(T.tris, first + j, a, b, c, id);
+                float t = tri_ray(a, b, c, o, d);
+                if (t < best || (t == best && bslot >= 0 && id < bid)) {
+                    best = t;
+                    bslot = first + j;
+                    bid = id;
+                    bound = best * BOUND_SLACK + 1e-30f;
+                }
+            }
+        }
+        ref = CHILD_EMPTY;
+        while (st.sp > 0) {
+            --st.sp;
+            if (st.key_at(st.sp) <= bound) {
+                ref = st.ref_at(st.sp);
+                break;
+            }
+        }
+        if (ref == CHILD_EMPTY) break;
+    }
+    best_t = best;
+    return bslot;
+}
+
+// true if some face is hit with t < MAX_DIST
+template <int BLOCK, int LDS_N>
+__device__ bool any_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
+    f3 inv = ray_inverse(d);
+    const float limit = MAX_DIST * BOUND_SLACK;
+    int32_t ref = 0;
+    st.sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            WideNode nd = load_node(T.nodes, ref);
+            float k[4];
+            int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
+            box_ray_4(nd, o, inv, limit, k);
+            sort4(k, r);
+            if (k[0] <= limit) {
+                if (k[3] <= limit) st.push(r[3], k[3]);
+                if (k[2] <= limit) st.push(r[2], k[2]);
+                if (k[1] <= limit) st.push(r[1], k[1]);
+                ref = r[0];
+                continue;
+            }
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, c;
+                int32_t id;
+                load_tri(T.tris, first + j, a, b, c, id);
+                if (tri_ray(a, b, c, o, d) < MAX_DIST) return true;
+            }
+        }
+        if (st.sp == 0) return false;
+        --st.sp;
+        ref = st.ref_at(st.sp);
+    }
+}
+
+// Mean unit normal of every face within squared distance NORMAL_EPS of point c
+// (src/bvh.cu:356-396).  The sum is accumulated in double so that it does not
+// depend on the visiting order, then rounded to float and divided by the count.
+template <int BLOCK, int LDS_N>
+__device__ f3 avg_normal(const Tree& T, f3 c, LaneStack<BLOCK, LDS_N>& st) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    float cnt = 0.0f;
+    int32_t ref = 0;
+    st.sp = 0;
+    for (;;) {
+        if (ref >= 0) {
+            WideNode nd = load_node(T.nodes, ref);
+            float k[4];
+            box_dist2_4(nd, c, k);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k[j] < NORMAL_EPS) st.push(nd.child[j], k[j]);
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, cc;
+                int32_t id;
+                load_tri(T.tris, first + j, a, b, cc, id);
+                if (tri_dist2(a, b, cc, c) < NORMAL_EPS) {
+                    f3 n = tri_normal(a, b, cc);
+                    sx += (double)n.x;
+                    sy += (double)n.y;
+                    sz += (double)n.z;
+                    cnt += 1.0f;
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        --st.sp;
+        ref = st.ref_at(st.sp);
+    }
+    return mk3((float)sx / cnt, (float)sy / cnt, (float)sz / cnt);
+}
+
+__device__ __forceinline__ f3 load_pt(const float* __restrict__ p, int64_t i) {
+    return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+__device__ __forceinline__ void store_pt(float* __restrict__ p, int64_t i, f3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// mode: 0 unsigned, 1 signed watertight, 2 signed raystab
+template <int BLOCK, int LDS_N, int MODE>
+__global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float* __restrict__ positions,
+                                                         float* __restrict__ distances, int64_t* __restrict__ face_id,
+                                                         float* __restrict__ uvw, Tree T) {
+    __shared__ int32_t s_ref[LDS_N * BLOCK];
+    __shared__ float s_key[LDS_N * BLOCK];
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    LaneStack<BLOCK, LDS_N> st;
+    st.ref = s_ref;
+    st.key = s_key;
+    st.tid = (int)threadIdx.x;
+    st.sp = 0;
+    const f3 p = load_pt(positions, i);
+    float d2;
+    int32_t bid;
+    int32_t slot = closest_face(T, p, st, d2, bid);
+    if (slot < 0) {  // nothing within MAX_DIST: distance 0 and face 0
+        slot = (int32_t)T.face0_slot;
+        d2 = 0.0f;
+    }
+    f3 a, b, c;
+    int32_t id;
+    load_tri(T.tris, slot, a, b, c, id);
+    float dist = sqrtf(d2);
+    if (MODE == 1) {
+        f3 cp = tri_closest_pt(a, b, c, p);
+        f3 nrm = avg_normal(T, cp, st);
+        dist = copysignf(dist, dot3(nrm, sub3(p, cp)));
+    } else if (MODE == 2) {
+        Pcg32 rng;
+        rng.advance((uint64_t)(uint32_t)((uint32_t)i * 2u));
+        float ox = rng.next_float();
+        float oy = rng.next_float();
+        bool inside = true;
+        for (uint32_t k = 0; k < 32; ++k) {
+            f3 dir = fibonacci_dir32(k, ox, oy);
+            if (!any_hit(T, p, neg3(dir), st) || !any_hit(T, p, dir, st)) {
+                inside = false;
+                break;
+            }
+        }
+        if (inside) dist = -dist;
+    }
+    distances[i] = dist;
+    face_id[i] = (int64_t)id;
+    if (uvw) {
+        f3 cp = tri_closest_pt(a, b, c, p);
+        store_pt(uvw, i, tri_barycentric(a, b, c, cp));
+    }
+}
+
+template <int BLOCK, int LDS_N>
+__global__ __launch_bounds__(BLOCK) void ray_kernel(int64_t n, const float* __restrict__ rays_o,
+                                                    const float* __restrict__ rays_d, float* __restrict__ positions,
+                                                    int64_t* __restrict__ face_id, float* __restrict__ depth, Tree T) {
+    __shared__ int32_t s_ref[LDS_N * BLOCK];
+    __shared__ float s_key[LDS_N * BLOCK];
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    LaneStack<BLOCK, LDS_N> st;
+    st.ref = s_ref;
+    st.key = s_key;
+    st.tid = (int)threadIdx.x;
+    st.sp = 0;
+    const f3 o = load_pt(rays_o, i);
+    const f3 d = load_pt(rays_d, i);
+    float t;
+    int32_t slot = closest_hit(T, o, d, st, t);
+    depth[i] = t;
+    store_pt(positions, i, mk3(__builtin_fmaf(t, d.x, o.x), __builtin_fmaf(t, d.y, o.y), __builtin_fmaf(t, d.z, o.z)));
+    int64_t fid = -1;
+    if (slot >= 0) fid = (int64_t)__float_as_int(T.tris[3 * slot].w);
+    face_id[i] = fid;
+}
+
+}  // namespace dev
+
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kLdsEntries = 12;
+
+hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
+    dev::Tree T{q.nodes, q.tris, q.face0_slot};
+    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
+    hipStream_t s = (hipStream_t)q.stream;
+    if (mode == 0)
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 0>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.out_f, q.out_id, q.out_v, T);
+    else if (mode == 1)
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 1>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.out_f, q.out_id, q.out_v, T);
+    else
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 2>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.out_f, q.out_id, q.out_v, T);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int stack_capacity_for(int need) { return need <= dev::STACK_TOTAL ? dev::STACK_TOTAL : -1; }
+
+hipError_t launch_distance(const QueryArgs& q, int mode) {
+    if (q.n <= 0) return hipSuccess;
+    if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
+    return launch_distance_impl(q, mode);
+}
+
+hipError_t launch_ray(const QueryArgs& q) {
+    if (q.n <= 0) return hipSuccess;
+    if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
+    dev::Tree T{q.nodes, q.tris, q.face0_slot};
+    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((dev::ray_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, (hipStream_t)q.stream, q.n,
+                       q.in0, q.in1, q.out_v, q.out_id, q.out_f, T);
+    return hipGetLastError();
+}
+
+}  // namespace cubvh_amd

@@ -1,0 +1,39 @@
+// query.hpp -- host-side launch interface of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh.hpp"
+
+namespace cubvh_amd {
+
+struct QueryArgs {
+    int64_t n = 0;
+    const float* in0 = nullptr;  // positions or ray origins [n,3]
+    const float* in1 = nullptr;  // ray directions [n,3]
+    float* out_f = nullptr;      // distances or depth [n]
+    int64_t* out_id = nullptr;   // face ids [n]
+    float* out_v = nullptr;      // uvw or hit positions [n,3]
+    const WideNode* nodes = nullptr;
+    const float4* tris = nullptr;
+    int64_t face0_slot = 0;
+    int32_t stack_need = 0;
+    void* stream = nullptr;
+};
+
+// Stack capacity instantiated for a tree that needs `need` entries (-1 if none).
+int stack_capacity_for(int need);
+
+// mode 0 unsigned distance, 1 signed (watertight), 2 signed (raystab)
+hipError_t launch_distance(const QueryArgs& q, int mode);
+hipError_t launch_ray(const QueryArgs& q);
+
+hipError_t launch_floodfill(const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D, int32_t* labels,
+                            hipStream_t stream);
+
+hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* positions, const int64_t* face_id,
+                               const float* uvw, const float* vertices, int64_t n_vertices, const int32_t* faces,
+                               int64_t n_faces, float* grad_points, float* grad_vertices, hipStream_t stream);
+
+}  // namespace cubvh_amd

@@ -1,0 +1,140 @@
+/*
+ * cubvh_amd.h -- C ABI of the MI355X (gfx950) BVH query library libcubvh_amd.so.
+ *
+ * Plain pointers and sizes only.  Every entry point returns CUBVH_OK (0) or a
+ * negative status; the message of the last failure on the calling thread is
+ * available from cubvh_last_error().  Nothing throws across this boundary and no
+ * entry point takes ownership of a caller pointer.
+ *
+ * Device entry points take device pointers that stay valid until the work queued
+ * on `stream` (a hipStream_t; NULL is the legacy default stream) has completed.
+ * They enqueue asynchronously on that stream and do not synchronize, except where
+ * noted.  `device` is the HIP ordinal the pointers live on.
+ *
+ * Reference interfaces replaced (ashawkey/cubvh):
+ *   src/bindings.cpp:14-26      pybind11 module _cubvh (cuBVH class, create_cuBVH,
+ *                               from_state_dict, floodfill)
+ *   include/gpu/api_gpu.h:17-32 abstract cubvh::cuBVH and its factories
+ *   src/api_gpu.cu:17-152       cuBVHImpl (gather, lazy upload, query methods)
+ */
+#ifndef CUBVH_AMD_H
+#define CUBVH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUBVH_OK 0
+#define CUBVH_ERR_ARG (-1)    /* invalid argument (null pointer, bad size, bad index) */
+#define CUBVH_ERR_HIP (-2)    /* a HIP runtime call failed */
+#define CUBVH_ERR_STATE (-3)  /* malformed serialized state */
+#define CUBVH_ERR_NOMEM (-4)  /* host allocation failed */
+
+/* Opaque BVH over one triangle mesh (host copy plus lazily uploaded device copies). */
+typedef struct cubvh_bvh cubvh_bvh;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* cubvh_last_error(void);
+
+/* ABI version: (major << 16) | minor. */
+int cubvh_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Construction (host only: no GPU is touched, as in the reference's lazy upload).
+ * Replaces create_cuBVH(vertices, triangles)   src/api_gpu.cu:128-130,
+ *          cuBVHImpl(vertices, triangles)      src/api_gpu.cu:21-44,
+ *          TriangleBvh4::build                 src/bvh.cu:515-620.
+ * vertices: fp32 [n_vertices, 3] row-major; faces: int32 [n_faces, 3] row-major.
+ * Face indices are bounds-checked (the reference does not check them).
+ * leaf_size: maximum triangles per leaf, 1..8 (0 selects the default, 4).
+ * ------------------------------------------------------------------------- */
+int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
+                     int32_t leaf_size, cubvh_bvh** out);
+
+/* Frees the host and every device copy.  Synchronizes the devices that hold one. */
+void cubvh_bvh_destroy(cubvh_bvh* bvh);
+
+/* Sizes of the serialized form and of the tree:
+ * info[0] = n_nodes (128-byte 4-wide nodes), info[1] = n_faces, info[2] = tree depth
+ * (root = 1), info[3] = stack entries the traversals need, info[4] = leaf size. */
+int cubvh_bvh_info(const cubvh_bvh* bvh, int64_t info[5]);
+
+/* ---------------------------------------------------------------------------
+ * Serialization.  Replaces cuBVH::state_dict / from_state_dict
+ * (include/gpu/bvh.cuh:84-109, src/api_gpu.cu:47-68,110-121).
+ * Own layout: nodes int32 [n_nodes, 32] (one 128-byte node per row) and
+ * triangles int32 [n_faces, 12] (the 48-byte record: a, b, c as fp32, 4 zero bytes,
+ * int64 original face index -- the same record layout as the reference).
+ * ------------------------------------------------------------------------- */
+int cubvh_bvh_export(const cubvh_bvh* bvh, int32_t* nodes, int32_t* triangles);
+int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* triangles, int64_t n_faces,
+                     cubvh_bvh** out);
+
+/* Imports a state_dict written by the reference: nodes int32 [n_nodes, 9]
+ * (TriangleBvhNode, bvh.cuh:14-21) and triangles int32 [n_faces, 12] (Triangle,
+ * triangle.cuh:174-175).  The triangles (with their original face ids) are rebuilt
+ * into this library's layout; the reference nodes are validated and then dropped. */
+int cubvh_bvh_import_reference(const int32_t* nodes9, int64_t n_nodes, const int32_t* triangles12,
+                               int64_t n_faces, int32_t leaf_size, cubvh_bvh** out);
+
+/* Mesh in original face order, recovered from the triangle records:
+ * tri_vertices fp32 [n_faces, 3, 3]. */
+int cubvh_bvh_triangles(const cubvh_bvh* bvh, float* tri_vertices);
+
+/* Copies the BVH to `device` now (otherwise done on the first query, as the
+ * reference's lazy upload at src/api_gpu.cu:76-78).  Synchronous. */
+int cubvh_bvh_upload(cubvh_bvh* bvh, int device, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Queries.  n is the number of queries.  face_id is the ORIGINAL face index
+ * (int64).  Ties are broken toward the lowest face index.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces cuBVH::unsigned_distance (src/api_gpu.cu:83-95, src/bvh.cu:468-488,
+ * unsigned_distance_kernel src/bvh.cu:687-712).
+ * positions fp32 [n,3]; distances fp32 [n]; face_id int64 [n]; uvw fp32 [n,3] or NULL.
+ * A query with no face within 1000 gets distance 0 and face 0. */
+int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* positions, float* distances,
+                            int64_t* face_id, float* uvw, void* stream);
+
+/* Replaces cuBVH::signed_distance (src/api_gpu.cu:97-108, src/bvh.cu:430-466,
+ * kernels src/bvh.cu:631-685).  mode 0 = watertight, any other value = raystab.
+ * The raystab random offsets are keyed by the query index within this call. */
+int cubvh_signed_distance(cubvh_bvh* bvh, int device, int64_t n, uint32_t mode, const float* positions,
+                          float* distances, int64_t* face_id, float* uvw, void* stream);
+
+/* Replaces cuBVH::ray_trace (src/api_gpu.cu:70-81, src/bvh.cu:490-513,
+ * raytrace_kernel src/bvh.cu:714-740).  rays_o, rays_d, positions fp32 [n,3];
+ * face_id int64 [n] (-1 on a miss); depth fp32 [n] (1000 on a miss). */
+int cubvh_ray_trace(cubvh_bvh* bvh, int device, int64_t n, const float* rays_o, const float* rays_d,
+                    float* positions, int64_t* face_id, float* depth, void* stream);
+
+/* Replaces floodfill (src/api_gpu.cu:136-152, _floodfill_batch
+ * include/gpu/floodfill.cuh:157-272).  grid: uint8/bool [B,H,W,D] (nonzero =
+ * occupied); labels int32 [B,H,W,D]: -1 for occupied voxels, otherwise the minimum
+ * linear index (global across the batch) of the voxel's 6-connected free component,
+ * in tensor order (last axis fastest).  B*H*W*D must be < 2^31.  Works in `labels`
+ * only (no scratch memory). */
+int cubvh_floodfill(int device, const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D,
+                    int32_t* labels, void* stream);
+
+/* No counterpart in the reference: backward of the unsigned distance.
+ * For query i with closest point c_i = uvw_i . (v_a, v_b, v_c) of face face_id[i]
+ * and n_i = (p_i - c_i) / |p_i - c_i| (zero when the distance is 0):
+ *   grad_points[i]   = grad_dist[i] * n_i
+ *   grad_vertices[v] += -grad_dist[i] * uvw_i[k] * n_i  for the k-th vertex v of the face
+ * vertices fp32 [n_vertices,3], faces int32 [n_faces,3] (original mesh, on device);
+ * grad_points fp32 [n,3] may be NULL; grad_vertices fp32 [n_vertices,3] may be NULL and
+ * is accumulated into (the caller zeroes it). */
+int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,
+                       const int64_t* face_id, const float* uvw, const float* vertices, int64_t n_vertices,
+                       const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBVH_AMD_H */

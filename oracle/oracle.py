@@ -1,0 +1,286 @@
+"""numpy wrappers over the CPU oracle (libcubvh_oracle*.so, built from cubvh_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product (cubvh_amd/) never
+imports it.  See cubvh_oracle.cpp for what each function restates and what pins it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_int64, c_uint32, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_lib = None
+
+
+def build(verbose: bool = False) -> None:
+    """Compile both oracle variants with the committed Makefile (g++)."""
+    r = subprocess.run(["make", "-B", "-C", str(HERE)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    if verbose:
+        print(r.stdout)
+
+
+def _cpu_has_fma() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " fma " in (line.rstrip() + " ")
+    except OSError:
+        pass
+    return False
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    name = "libcubvh_oracle_fma.so" if _cpu_has_fma() else "libcubvh_oracle.so"
+    path = HERE / name
+    if not path.exists():
+        build()
+    lib = ctypes.CDLL(str(path))
+    vp, i64 = c_void_p, c_int64
+    sig = {
+        "orc_num_threads": (c_int, []),
+        "orc_tri_dist2": (None, [vp, vp, i64, vp]),
+        "orc_tri_closest": (None, [vp, vp, i64, vp, vp]),
+        "orc_tri_ray": (None, [vp, vp, vp, i64, vp, c_int]),
+        "orc_safe_divide": (c_float, [c_float, c_float]),
+        "orc_pcg32_uints": (None, [i64, i64, vp]),
+        "orc_pcg32_offsets": (None, [i64, vp]),
+        "orc_fibonacci_dirs32": (None, [c_float, c_float, vp]),
+        "orc_udf_brute": (c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp, c_int]),
+        "orc_ray_brute": (c_int, [vp, i64, vp, i64, vp, vp, i64, vp, vp, vp, c_int]),
+        "orc_sdf_brute": (c_int, [vp, i64, vp, i64, vp, i64, c_uint32, vp, vp, vp, c_int]),
+        "orc_udf_f64": (c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp, vp, c_int]),
+        "orc_ray_f64": (c_int, [vp, i64, vp, i64, vp, vp, i64, c_double, vp, vp, vp, c_int]),
+        "orc_winding_f64": (c_int, [vp, i64, vp, i64, vp, i64, vp, c_int]),
+        "orc_refbvh_build": (vp, [vp, i64, vp, i64]),
+        "orc_refbvh_free": (None, [vp]),
+        "orc_refbvh_num_nodes": (i64, [vp]),
+        "orc_refbvh_export": (None, [vp, vp, vp]),
+        "orc_refbvh_udf": (c_int, [vp, vp, i64, vp, vp, vp, c_int, vp, c_int]),
+        "orc_refbvh_ray": (c_int, [vp, vp, vp, i64, vp, vp, vp, c_int, vp, c_int]),
+        "orc_refbvh_sdf": (c_int, [vp, vp, i64, c_uint32, vp, vp, vp, vp, c_int]),
+        "orc_floodfill_ref": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
+        "orc_floodfill_bfs": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
+    }
+    for k, (res, args) in sig.items():
+        fn = getattr(lib, k)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def num_threads() -> int:
+    return load().orc_num_threads()
+
+
+# ---------------------------------------------------------------- primitives
+def tri_dist2(tri, pts):
+    tri, pts = _f32(tri).reshape(9), _f32(pts).reshape(-1, 3)
+    out = np.empty(len(pts), np.float32)
+    load().orc_tri_dist2(_p(tri), _p(pts), len(pts), _p(out))
+    return out
+
+
+def tri_closest(tri, pts):
+    tri, pts = _f32(tri).reshape(9), _f32(pts).reshape(-1, 3)
+    cp = np.empty_like(pts)
+    uvw = np.empty_like(pts)
+    load().orc_tri_closest(_p(tri), _p(pts), len(pts), _p(cp), _p(uvw))
+    return cp, uvw
+
+
+def tri_ray(tri, ro, rd, exact_div=True):
+    tri, ro, rd = _f32(tri).reshape(9), _f32(ro).reshape(-1, 3), _f32(rd).reshape(-1, 3)
+    out = np.empty(len(ro), np.float32)
+    load().orc_tri_ray(_p(tri), _p(ro), _p(rd), len(ro), _p(out), int(exact_div))
+    return out
+
+
+def safe_divide(a, b):
+    return load().orc_safe_divide(float(a), float(b))
+
+
+def pcg32_uints(advance, n):
+    out = np.empty(n, np.uint32)
+    load().orc_pcg32_uints(int(advance), int(n), _p(out))
+    return out
+
+
+def pcg32_offsets(n):
+    out = np.empty((n, 2), np.float32)
+    load().orc_pcg32_offsets(int(n), _p(out))
+    return out
+
+
+def fibonacci_dirs32(ox, oy):
+    out = np.empty((32, 3), np.float32)
+    load().orc_fibonacci_dirs32(float(ox), float(oy), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- brute force
+def udf_brute(V, F, pts, return_uvw=True, nthreads=0):
+    V, F, pts = _f32(V), _i32(F), _f32(pts).reshape(-1, 3)
+    n = len(pts)
+    d = np.empty(n, np.float32)
+    f = np.empty(n, np.int64)
+    uvw = np.empty((n, 3), np.float32) if return_uvw else None
+    rc = load().orc_udf_brute(_p(V), len(V), _p(F), len(F), _p(pts), n, _p(d), _p(f), _p(uvw), nthreads)
+    assert rc == 0
+    return d, f, uvw
+
+
+def ray_brute(V, F, ro, rd, nthreads=0):
+    V, F, ro, rd = _f32(V), _i32(F), _f32(ro).reshape(-1, 3), _f32(rd).reshape(-1, 3)
+    n = len(ro)
+    pos = np.empty((n, 3), np.float32)
+    f = np.empty(n, np.int64)
+    t = np.empty(n, np.float32)
+    rc = load().orc_ray_brute(_p(V), len(V), _p(F), len(F), _p(ro), _p(rd), n, _p(pos), _p(f), _p(t), nthreads)
+    assert rc == 0
+    return pos, f, t
+
+
+def sdf_brute(V, F, pts, mode, return_uvw=True, nthreads=0):
+    V, F, pts = _f32(V), _i32(F), _f32(pts).reshape(-1, 3)
+    n = len(pts)
+    d = np.empty(n, np.float32)
+    f = np.empty(n, np.int64)
+    uvw = np.empty((n, 3), np.float32) if return_uvw else None
+    rc = load().orc_sdf_brute(_p(V), len(V), _p(F), len(F), _p(pts), n, int(mode), _p(d), _p(f), _p(uvw), nthreads)
+    assert rc == 0
+    return d, f, uvw
+
+
+def udf_f64(V, F, pts, nthreads=0):
+    V, F, pts = _f32(V), _i32(F), _f32(pts).reshape(-1, 3)
+    n = len(pts)
+    d = np.empty(n, np.float64)
+    f = np.empty(n, np.int64)
+    cp = np.empty((n, 3), np.float64)
+    gap = np.empty(n, np.float64)
+    rc = load().orc_udf_f64(_p(V), len(V), _p(F), len(F), _p(pts), n, _p(d), _p(f), _p(cp), _p(gap), nthreads)
+    assert rc == 0
+    return d, f, cp, gap
+
+
+def ray_f64(V, F, ro, rd, tol=1e-6, nthreads=0):
+    V, F, ro, rd = _f32(V), _i32(F), _f32(ro).reshape(-1, 3), _f32(rd).reshape(-1, 3)
+    n = len(ro)
+    t = np.empty(n, np.float64)
+    f = np.empty(n, np.int64)
+    amb = np.empty(n, np.uint8)
+    rc = load().orc_ray_f64(_p(V), len(V), _p(F), len(F), _p(ro), _p(rd), n, float(tol), _p(t), _p(f), _p(amb),
+                            nthreads)
+    assert rc == 0
+    return t, f, amb.astype(bool)
+
+
+def winding_f64(V, F, pts, nthreads=0):
+    V, F, pts = _f32(V), _i32(F), _f32(pts).reshape(-1, 3)
+    w = np.empty(len(pts), np.float64)
+    rc = load().orc_winding_f64(_p(V), len(V), _p(F), len(F), _p(pts), len(pts), _p(w), nthreads)
+    assert rc == 0
+    return w
+
+
+# ---------------------------------------------------------------- reference BVH
+class RefBvh:
+    """The reference's BVH4 (src/bvh.cu:515-620) and its traversals."""
+
+    def __init__(self, V, F):
+        self.V, self.F = _f32(V), _i32(F)
+        self.h = load().orc_refbvh_build(_p(self.V), len(self.V), _p(self.F), len(self.F))
+        if not self.h:
+            raise ValueError("bad mesh")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().orc_refbvh_free(self.h)
+            self.h = None
+
+    @property
+    def n_nodes(self):
+        return load().orc_refbvh_num_nodes(self.h)
+
+    def export(self):
+        """Reference state_dict arrays: nodes int32 [n, 9], triangles int32 [M, 12]."""
+        nodes = np.empty((self.n_nodes, 9), np.int32)
+        tris = np.empty((len(self.F), 12), np.int32)
+        load().orc_refbvh_export(self.h, _p(nodes), _p(tris))
+        return nodes, tris
+
+    def udf(self, pts, return_uvw=True, tie_mode=0, nthreads=0):
+        pts = _f32(pts).reshape(-1, 3)
+        n = len(pts)
+        d = np.empty(n, np.float32)
+        f = np.empty(n, np.int64)
+        uvw = np.empty((n, 3), np.float32) if return_uvw else None
+        counts = np.zeros(2, np.int64)
+        load().orc_refbvh_udf(self.h, _p(pts), n, _p(d), _p(f), _p(uvw), tie_mode, _p(counts), nthreads)
+        return d, f, uvw, counts
+
+    def ray(self, ro, rd, tie_mode=0, nthreads=0):
+        ro, rd = _f32(ro).reshape(-1, 3), _f32(rd).reshape(-1, 3)
+        n = len(ro)
+        pos = np.empty((n, 3), np.float32)
+        f = np.empty(n, np.int64)
+        t = np.empty(n, np.float32)
+        counts = np.zeros(2, np.int64)
+        load().orc_refbvh_ray(self.h, _p(ro), _p(rd), n, _p(pos), _p(f), _p(t), tie_mode, _p(counts), nthreads)
+        return pos, f, t, counts
+
+    def sdf(self, pts, mode, return_uvw=True, nthreads=0):
+        pts = _f32(pts).reshape(-1, 3)
+        n = len(pts)
+        d = np.empty(n, np.float32)
+        f = np.empty(n, np.int64)
+        uvw = np.empty((n, 3), np.float32) if return_uvw else None
+        counts = np.zeros(4, np.int64)
+        load().orc_refbvh_sdf(self.h, _p(pts), n, int(mode), _p(d), _p(f), _p(uvw), _p(counts), nthreads)
+        return d, f, uvw, counts
+
+
+# ---------------------------------------------------------------- floodfill
+def floodfill_ref(grid):
+    g = np.ascontiguousarray(grid, dtype=np.uint8)
+    if g.ndim == 3:
+        g = g[None]
+    B, H, W, D = g.shape
+    out = np.empty(g.shape, np.int32)
+    it = load().orc_floodfill_ref(_p(g), B, H, W, D, _p(out))
+    return out, it
+
+
+def floodfill_bfs(grid):
+    g = np.ascontiguousarray(grid, dtype=np.uint8)
+    if g.ndim == 3:
+        g = g[None]
+    B, H, W, D = g.shape
+    out = np.empty(g.shape, np.int32)
+    load().orc_floodfill_bfs(_p(g), B, H, W, D, _p(out))
+    return out

@@ -1,0 +1,201 @@
+"""The C ABI library without a GPU: symbols, host build, serialization, errors.
+
+Nothing here launches a kernel; construction and state_dict are host-only, like the
+reference's lazy GPU initialisation (test/lazy_gpu_initialization.py:94-105).
+"""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "cubvh_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cubvh_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes table out of sync with the header"
+    assert lib.cubvh_abi_version() >> 16 == 1
+
+
+def test_create_info_export_import_roundtrip(ico4):
+    import cubvh_amd
+
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    info = bvh.info()
+    assert info["n_faces"] == len(f)
+    assert info["stack_need"] == 3 * info["depth"] + 1 <= 64
+    sd = bvh.state_dict()
+    assert sd["nodes"].dtype == torch.int32 and sd["nodes"].shape == (info["n_nodes"], 32)
+    assert sd["triangles"].shape == (len(f), 12)
+    # triangle rows carry the 48-byte record: a, b, c, 4 zero bytes, int64 face id
+    rec = sd["triangles"].numpy()
+    ids = rec.view(np.int64)[:, 5]
+    assert sorted(ids.tolist()) == list(range(len(f)))
+    assert (rec[:, 9] == 0).all()
+    abc = rec[:, :9].view(np.float32).reshape(-1, 3, 3)
+    np.testing.assert_array_equal(abc, v[f[ids]])
+    b2 = cubvh_amd.cuBVH.from_state_dict(sd)
+    sd2 = b2.state_dict()
+    for k in sd:
+        assert torch.equal(sd[k], sd2[k])
+
+
+def test_state_dict_survives_torch_save(tmp_path, ico4):
+    import cubvh_amd
+
+    v, f = ico4
+    sd = cubvh_amd.cuBVH(v, f).state_dict()
+    torch.save(sd, tmp_path / "bvh.pt")
+    sd2 = torch.load(tmp_path / "bvh.pt", weights_only=True)
+    b = cubvh_amd.cuBVH.from_state_dict(sd2)
+    assert b.info()["n_faces"] == len(f)
+
+
+def test_import_reference_state_dict(oracle, ico4):
+    """A state_dict in the reference's layout (nodes [n, 9], triangles [M, 12]) is
+    accepted; the mesh recovered from it equals the original faces."""
+    import cubvh_amd
+
+    v, f = ico4
+    nodes9, tris12 = oracle.RefBvh(v, f).export()
+    sd = {"nodes": torch.from_numpy(nodes9), "triangles": torch.from_numpy(tris12)}
+    b = cubvh_amd.cuBVH.from_state_dict(sd)
+    tv = b._vertices.reshape(-1, 3, 3)
+    np.testing.assert_array_equal(tv, v[f])
+
+
+def test_reference_state_dict_validation(oracle, ico4):
+    import cubvh_amd
+    from cubvh_amd._lib import CubvhError
+
+    v, f = ico4
+    nodes9, tris12 = oracle.RefBvh(v, f).export()
+    bad = tris12.copy()
+    bad.view(np.int64)[3, 5] = bad.view(np.int64)[4, 5]  # duplicate face id
+    with pytest.raises(CubvhError, match="permutation"):
+        cubvh_amd.cuBVH.from_state_dict({"nodes": torch.from_numpy(nodes9), "triangles": torch.from_numpy(bad)})
+    badn = nodes9.copy()
+    badn[0, 7] = 10**6  # child range past the end
+    with pytest.raises(CubvhError, match="child range"):
+        cubvh_amd.cuBVH.from_state_dict({"nodes": torch.from_numpy(badn), "triangles": torch.from_numpy(tris12)})
+    with pytest.raises(RuntimeError, match="missing"):
+        cubvh_amd.cuBVH.from_state_dict({"nodes": torch.from_numpy(nodes9)})
+
+
+def test_own_state_dict_validation(ico4):
+    import cubvh_amd
+    from cubvh_amd._lib import CubvhError
+
+    v, f = ico4
+    sd = cubvh_amd.cuBVH(v, f).state_dict()
+    nodes = sd["nodes"].clone()
+    nodes[0, 24] = 10**7  # child reference out of range
+    with pytest.raises(CubvhError, match="invalid state"):
+        cubvh_amd.cuBVH.from_state_dict({"nodes": nodes, "triangles": sd["triangles"]})
+
+
+def test_bad_meshes_raise(ico4):
+    import cubvh_amd
+
+    v, f = ico4
+    with pytest.raises(AssertionError, match="at least 8"):
+        cubvh_amd.cuBVH(v, f[:8])
+    bad = f.copy()
+    bad[10, 1] = len(v)
+    with pytest.raises(ValueError, match="outside"):
+        cubvh_amd.cuBVH(v, bad)
+    with pytest.raises(ValueError):
+        cubvh_amd.cuBVH(v[:, :2], f)
+
+
+def test_c_abi_rejects_bad_face_index_with_message():
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    v = np.zeros((4, 3), np.float32)
+    f = np.array([[0, 1, 5]] * 9, np.int32)
+    h = ctypes.c_void_p()
+    rc = lib.cubvh_bvh_create(v.ctypes.data, 4, f.ctypes.data, 9, 0, ctypes.byref(h))
+    assert rc == -1
+    assert b"references vertex 5" in lib.cubvh_last_error()
+    rc = lib.cubvh_bvh_create(v.ctypes.data, 4, f.ctypes.data, 9, 9, ctypes.byref(h))
+    assert rc == -1
+    # non-finite coordinates are refused
+    v[0, 0] = np.nan
+    f = np.array([[0, 1, 2]] * 9, np.int32)
+    rc = lib.cubvh_bvh_create(v.ctypes.data, 4, f.ctypes.data, 9, 0, ctypes.byref(h))
+    assert rc == -1 and b"finite" in lib.cubvh_last_error()
+
+
+def test_float64_and_int64_inputs_are_cast(ico4):
+    """The reference's caster rounds geometry to fp32 (patch/eigen/matrix.h:490-524)."""
+    import cubvh_amd
+
+    v, f = ico4
+    a = cubvh_amd.cuBVH(v.astype(np.float64), f.astype(np.int64)).state_dict()
+    b = cubvh_amd.cuBVH(torch.from_numpy(v), torch.from_numpy(f.astype(np.int64))).state_dict()
+    for k in a:
+        assert torch.equal(a[k], b[k])
+
+
+def test_tree_properties_large_mesh():
+    """Level-6 icosphere: every face in exactly one leaf, boxes contain their faces."""
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    v, f = S.jittered_sphere(6)
+    bvh = cubvh_amd.cuBVH(v, f)
+    sd = bvh.state_dict()
+    nodes = sd["nodes"].numpy()
+    rec = sd["triangles"].numpy()
+    abc = rec[:, :9].view(np.float32).reshape(-1, 3, 3)
+    bounds = nodes[:, :24].view(np.float32).reshape(-1, 6, 4)  # lo_x, hi_x, lo_y, hi_y, lo_z, hi_z
+    child = nodes[:, 24:28]
+    seen = np.zeros(len(f), np.int64)
+    for ni in range(len(nodes)):
+        for k in range(4):
+            r = child[ni, k]
+            if r == -1 or r >= 0:
+                continue
+            u = np.uint32(r & 0x7FFFFFFF)
+            first, cnt = int(u >> 3), int(u & 7) + 1
+            seen[first:first + cnt] += 1
+            tri = abc[first:first + cnt]
+            lo = bounds[ni, 0::2, k]
+            hi = bounds[ni, 1::2, k]
+            assert (tri.min(axis=(0, 1)) >= lo).all() and (tri.max(axis=(0, 1)) <= hi).all()
+    assert (seen == 1).all()
+
+
+def test_reference_package_names_exist():
+    import cubvh_amd
+
+    for name in ("cuBVH", "HashTable", "cuHashTable", "sparse_marching_cubes", "sparse_marching_cubes_cpu",
+                 "floodfill", "fill_holes", "merge_vertices", "decimate", "parallel_decimate"):
+        assert hasattr(cubvh_amd, name)
+    with pytest.raises(NotImplementedError):
+        cubvh_amd.fill_holes(None, None)
+    with pytest.raises(NotImplementedError):
+        cubvh_amd.cuHashTable()
+
+
+def test_floodfill_requires_bool_grid():
+    import cubvh_amd
+
+    with pytest.raises(TypeError, match="bool"):
+        cubvh_amd.floodfill(torch.zeros(4, 4, 4, dtype=torch.uint8))

@@ -1,0 +1,96 @@
+"""Backward of the unsigned distance (cubvh_udf_backward) against FP64 central finite
+differences (h = 1e-4) and against an FP64 torch reference of the same closed form.
+
+Tolerances: analytic vs FD |g - g_fd| <= 2e-3 * max(|g_fd|, 1e-3 scale of the loss)
+on coordinates whose query is not near a face switch (FP64 gap to the second face
+> 1e-3, distance > 1e-3); kernel vs FP64 torch reference within 1e-5 relative."""
+import numpy as np
+import pytest
+import torch
+
+import cubvh_amd
+from cubvh_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def test_point_and_vertex_gradients_vs_finite_differences(gpu, oracle):
+    v, f = S.jittered_sphere(3, seed=1)  # 1,280 faces, small enough for FP64 brute force
+    pts = S.normal_points(2048, 0.4, seed=3)
+    n = len(pts)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu).requires_grad_(True)
+    Vt = torch.from_numpy(v).to(gpu).requires_grad_(True)
+    d, fid, uvw = cubvh_amd.udf(bvh, P, Vt)
+    loss = (d * d).mean()
+    loss.backward()
+    gp = P.grad.cpu().numpy().astype(np.float64)
+    gv = Vt.grad.cpu().numpy().astype(np.float64)
+
+    d64, f64, cp64, gap = oracle.udf_f64(v, f, pts)
+    good = (gap > 1e-3) & (d64 > 1e-3)
+    assert good.sum() > 1500
+    h = 1e-4
+
+    # points: dL/dp_i = 2 d_i dd_i/dp_i / n
+    rng = np.random.default_rng(0)
+    idx = rng.choice(np.nonzero(good)[0], 300, replace=False)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        dp, _, _, _ = oracle.udf_f64(v, f, pts[idx] + e)
+        dm, _, _, _ = oracle.udf_f64(v, f, pts[idx] - e)
+        fd = (dp * dp - dm * dm) / (2 * h) / n
+        np.testing.assert_allclose(gp[idx, k], fd, rtol=2e-3, atol=2e-3 / n)
+
+    # vertices: FD of the whole loss over the well-conditioned queries
+    sub = pts[good]
+    ng = len(sub)
+    Psub = torch.from_numpy(sub).to(gpu)
+    Vt2 = torch.from_numpy(v).to(gpu).requires_grad_(True)
+    d2, _, _ = cubvh_amd.udf(bvh, Psub, Vt2)
+    ((d2 * d2).sum() / ng).backward()
+    gv2 = Vt2.grad.cpu().numpy().astype(np.float64)
+    used = np.unique(f[f64[good]])
+    coords = rng.choice(len(used) * 3, 64, replace=False)
+    for c in coords:
+        vi, k = used[c // 3], c % 3
+        vp, vm = v.astype(np.float64).copy(), v.astype(np.float64).copy()
+        vp[vi, k] += h
+        vm[vi, k] -= h
+        lp = (oracle.udf_f64(vp.astype(np.float32), f, sub)[0] ** 2).sum() / ng
+        lm = (oracle.udf_f64(vm.astype(np.float32), f, sub)[0] ** 2).sum() / ng
+        # vertices are fp32: use the actually applied step
+        hp = np.float64(np.float32(vp[vi, k])) - np.float64(v[vi, k])
+        hm = np.float64(v[vi, k]) - np.float64(np.float32(vm[vi, k]))
+        fd = (lp - lm) / (hp + hm)
+        assert abs(gv2[vi, k] - fd) <= 3e-3 * max(abs(fd), 1e-3), (vi, k, gv2[vi, k], fd)
+
+
+def test_gradients_vs_fp64_torch_reference(gpu, oracle):
+    """Kernel output equals the closed form n = (p - c)/|p - c|, grad_v = -g w n
+    evaluated in FP64 with torch (index_add) from the kernel's own face ids and uvw."""
+    v, f = S.jittered_sphere(6, seed=0)
+    pts = S.normal_points(1 << 16, 0.4, seed=1)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu).requires_grad_(True)
+    Vt = torch.from_numpy(v).to(gpu).requires_grad_(True)
+    d, fid, uvw = cubvh_amd.udf(bvh, P, Vt)
+    g = torch.linspace(-1, 1, len(pts), device=gpu)
+    (d * g).sum().backward()
+    Vd = Vt.detach().double()
+    F = torch.from_numpy(f).to(gpu).long()
+    tri = F[fid]
+    w = uvw.double()
+    c = (Vd[tri] * w[..., None]).sum(1)
+    diff = P.detach().double() - c
+    nrm = diff / diff.norm(dim=1, keepdim=True)
+    ref_p = g.double()[:, None] * nrm
+    ref_v = torch.zeros_like(Vd)
+    for k in range(3):
+        ref_v.index_add_(0, tri[:, k], -(g.double() * w[:, k])[:, None] * nrm)
+    torch.testing.assert_close(P.grad.double(), ref_p, rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(Vt.grad.double(), ref_v, rtol=1e-4, atol=1e-5)
+    # forward values unchanged by the autograd wrapper
+    d0, f0, u0 = bvh.unsigned_distance(P.detach(), return_uvw=True)
+    assert torch.equal(d0, d.detach()) and torch.equal(f0, fid) and torch.equal(u0, uvw)

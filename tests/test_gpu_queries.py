@@ -1,0 +1,200 @@
+"""GPU parity of the BVH queries (HIP kernels in libcubvh_amd.so) against the oracle.
+
+Bar (tolerances written in each test):
+  * against the oracle's fp32 brute force (same formulas, canonical ties to the
+    lowest face index): distances, face ids, uvw, depths, hit positions and signs
+    are bit-exact;
+  * against FP64: |d - d64| <= 1e-5 * max(d64, 1); hit/miss and clear-winner face
+    ids exact; closest point rebuilt from face_id + uvw within 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+import cubvh_amd
+from cubvh_amd import synthetic as S
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_udf_small_workload_bit_exact(gpu, oracle, ico4):
+    v, f = ico4
+    p = S.uniform_points(65536, seed=0)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, uvw = bvh.unsigned_distance(torch.from_numpy(p).to(gpu), return_uvw=True)
+    assert fi.dtype == torch.int64 and d.dtype == torch.float32 and uvw.shape == (65536, 3)
+    od, of, ou = oracle.udf_brute(v, f, p)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(uvw), ou)
+    # committed golden answers (first 4096 points), so a changed oracle cannot hide a change
+    g = np.load(GOLDEN / "config1_subset.npz")
+    np.testing.assert_array_equal(_np(d)[:4096], g["dist"])
+    np.testing.assert_array_equal(_np(fi)[:4096], g["face"])
+    np.testing.assert_array_equal(_np(uvw)[:4096], g["uvw"])
+
+
+def test_udf_vs_fp64(gpu, oracle, ico4):
+    v, f = ico4
+    p = S.uniform_points(16384, seed=11)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, uvw = bvh.unsigned_distance(torch.from_numpy(p).to(gpu), return_uvw=True)
+    d, fi, uvw = _np(d).astype(np.float64), _np(fi), _np(uvw).astype(np.float64)
+    d64, f64, cp64, gap = oracle.udf_f64(v, f, p)
+    err = np.abs(d - d64) / np.maximum(d64, 1.0)
+    assert err.max() <= 1e-5, err.max()
+    print(f"max |d - d64| / max(d64, 1) = {err.max():.3g}")
+    clear = gap > 1e-5 * np.maximum(d64, 1.0)
+    assert (fi[clear] == f64[clear]).all()
+    vd = v.astype(np.float64)
+    cp = sum(vd[f[fi, k]] * uvw[:, [k]] for k in range(3))
+    assert np.abs(cp - cp64)[clear].max() <= 1e-5
+    assert np.abs(np.linalg.norm(p - cp, axis=1) - d64).max() <= 1e-5
+
+
+def test_udf_reference_fixture_dodecahedron(gpu, oracle, dodecahedron):
+    """The reference's own check (test/unsigned_distance.py:93-139): 1000 randn points
+    on the dodecahedron, distance and rebuilt closest point within atol 1e-5."""
+    v, f = dodecahedron
+    p = np.random.default_rng(0).normal(size=(1000, 3)).astype(np.float32)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, uvw = bvh.unsigned_distance(torch.from_numpy(p), return_uvw=True)  # CPU input is moved to the GPU
+    assert d.is_cuda
+    d64, _, cp64, gap = oracle.udf_f64(v, f, p)
+    np.testing.assert_allclose(_np(d), d64, atol=1e-5)
+    fi, uvw = _np(fi), _np(uvw).astype(np.float64)
+    vd = v.astype(np.float64)
+    cp = sum(vd[f[fi, k]] * uvw[:, [k]] for k in range(3))
+    clear = gap > 1e-5
+    np.testing.assert_allclose(cp[clear], cp64[clear], atol=1e-5)
+
+
+def test_udf_leading_dims_and_far_points(gpu, oracle, ico4):
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    p = torch.from_numpy(S.uniform_points(2 * 3 * 50, seed=5).reshape(2, 3, 50, 3)).double()
+    d, fi, uvw = bvh.unsigned_distance(p.to(gpu), return_uvw=True)
+    assert d.shape == (2, 3, 50) and fi.shape == (2, 3, 50) and uvw.shape == (2, 3, 50, 3)
+    d2, _, _ = bvh.unsigned_distance(p.reshape(-1, 3).to(gpu))
+    assert torch.equal(d.reshape(-1), d2)
+    # nothing within 1000: distance 0 and face 0 (src/bvh.cu:346-350 returns slot 0)
+    far = torch.tensor([[5000.0, 0, 0], [0, -3000.0, 0]], device=gpu)
+    d, fi, _ = bvh.unsigned_distance(far)
+    assert _np(d).tolist() == [0.0, 0.0] and _np(fi).tolist() == [0, 0]
+    # empty query set
+    d, fi, uvw = bvh.unsigned_distance(torch.zeros(0, 3, device=gpu), return_uvw=True)
+    assert d.shape == (0,) and uvw.shape == (0, 3)
+
+
+def test_udf_perturbed_level6_bit_exact(gpu, oracle):
+    """A deeper tree (81,920 faces) against the oracle's reference-BVH walk with
+    canonical ties (itself equal to the brute force, see test_oracle_bvh)."""
+    v, f = S.jittered_sphere(6)
+    p = S.normal_points(20000, 0.4, seed=2)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, uvw = bvh.unsigned_distance(torch.from_numpy(p).to(gpu), return_uvw=True)
+    R = oracle.RefBvh(v, f)
+    od, of, ou, _ = R.udf(p, tie_mode=1)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(uvw), ou)
+
+
+def test_rays_bit_exact_and_vs_fp64(gpu, oracle, ico4):
+    v, f = ico4
+    ro, rd = S.sphere_rays(65536, seed=0)
+    bvh = cubvh_amd.cuBVH(v, f)
+    pos, fi, t = bvh.ray_trace(torch.from_numpy(ro).to(gpu), torch.from_numpy(rd).to(gpu))
+    opos, of, ot = oracle.ray_brute(v, f, ro, rd)
+    np.testing.assert_array_equal(_np(t), ot)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(pos), opos)
+    g = np.load(GOLDEN / "config1_subset.npz")
+    np.testing.assert_array_equal(_np(t)[:4096], g["ray_t"])
+    np.testing.assert_array_equal(_np(fi)[:4096], g["ray_face"])
+    t64, f64, amb = oracle.ray_f64(v, f, ro, rd, tol=1e-5)
+    ok = ~amb
+    fi, t = _np(fi), _np(t)
+    assert ((fi >= 0) == (f64 >= 0))[ok].all()
+    hit = ok & (f64 >= 0)
+    assert (fi[hit] == f64[hit]).all()
+    np.testing.assert_allclose(t[hit], t64[hit], rtol=1e-5)
+    assert (t[fi < 0] == 1000.0).all()
+
+
+def test_rays_miss_and_shapes(gpu, ico4):
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    o = torch.tensor([[[0.0, 0.0, 3.0], [0.0, 0.0, 3.0]]], device=gpu)
+    d = torch.tensor([[[0.0, 0.0, -1.0], [0.0, 1.0, 0.0]]], device=gpu)
+    pos, fi, t = bvh.ray_trace(o, d)
+    assert pos.shape == (1, 2, 3) and fi.shape == (1, 2)
+    assert _np(fi)[0, 1] == -1 and _np(t)[0, 1] == 1000.0
+    np.testing.assert_allclose(_np(pos)[0, 1], [0, 1000.0, 3.0])
+    assert _np(fi)[0, 0] >= 0 and abs(_np(t)[0, 0] - 2.5) < 2e-3  # sphere of radius 0.5
+    # axis-parallel rays (zero direction components) are handled
+    o = torch.tensor([[0.01, 0.02, -2.0], [2.0, 0.0, 0.0]], device=gpu)
+    d = torch.tensor([[0.0, 0.0, 1.0], [-1.0, 0.0, 0.0]], device=gpu)
+    _, fi, t = bvh.ray_trace(o, d)
+    assert (_np(fi) >= 0).all()
+    np.testing.assert_allclose(_np(t), [1.5, 1.5], atol=5e-3)
+
+
+def test_sdf_watertight_bit_exact(gpu, oracle, ico4):
+    v, f = ico4
+    p = S.uniform_points(32768, seed=6)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, uvw = bvh.signed_distance(torch.from_numpy(p).to(gpu), return_uvw=True, mode="watertight")
+    od, of, ou = oracle.sdf_brute(v, f, p, 0)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(uvw), ou)
+    w = oracle.winding_f64(v, f, p)
+    far = np.abs(od) > 1e-4
+    assert ((_np(d) < 0) == (w > 0.5))[far].all()
+
+
+def test_sdf_raystab_bit_exact_closed_and_soup(gpu, oracle, ico4):
+    v, f = ico4
+    p = S.uniform_points(8192, seed=8)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d, fi, _ = bvh.signed_distance(torch.from_numpy(p).to(gpu), mode="raystab")
+    od, of, _ = oracle.sdf_brute(v, f, p, 1, return_uvw=False)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    w = oracle.winding_f64(v, f, p)
+    assert ((_np(d) < 0) == (w > 0.5))[np.abs(od) > 1e-4].all()
+    # non-watertight soup: level-4 sphere minus 20% faces plus small random triangles
+    rng = np.random.default_rng(3)
+    keep = f[np.sort(rng.permutation(len(f))[: int(0.8 * len(f))])]
+    c = rng.uniform(-0.6, 0.6, size=(500, 3))
+    tv = (c[:, None, :] + rng.normal(0, 0.02, size=(500, 3, 3))).reshape(-1, 3)
+    V2 = np.concatenate([v, tv]).astype(np.float32)
+    F2 = np.concatenate([keep, len(v) + np.arange(1500).reshape(-1, 3)]).astype(np.int32)
+    bvh2 = cubvh_amd.cuBVH(V2, F2)
+    d, fi, uvw = bvh2.signed_distance(torch.from_numpy(p).to(gpu), return_uvw=True, mode="raystab")
+    od, of, ou = oracle.sdf_brute(V2, F2, p, 1)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(uvw), ou)
+    assert (od < 0).any() and (od > 0).any()
+
+
+def test_state_dict_roundtrip_on_gpu(gpu, oracle, ico4):
+    """test/state_dict.py:96-112: save, load, from_state_dict give identical results;
+    the reference's own layout loads too."""
+    v, f = ico4
+    p = torch.from_numpy(S.uniform_points(4096, seed=9)).to(gpu)
+    bvh = cubvh_amd.cuBVH(v, f)
+    a = bvh.unsigned_distance(p, return_uvw=True)
+    b = cubvh_amd.cuBVH.from_state_dict(bvh.state_dict()).unsigned_distance(p, return_uvw=True)
+    nodes9, tris12 = oracle.RefBvh(v, f).export()
+    ref = cubvh_amd.cuBVH.from_state_dict({"nodes": torch.from_numpy(nodes9), "triangles": torch.from_numpy(tris12)})
+    c = ref.unsigned_distance(p, return_uvw=True)
+    for x, y, z in zip(a, b, c):
+        assert torch.equal(x, y) and torch.equal(x, z)
