@@ -31,6 +31,25 @@ int fail_hip(hipError_t e, const char* what) {
 }
 
 constexpr int kMaxDevices = 64;
+// Query batches at least this large are sorted into Morton order before the walk.
+constexpr int64_t kMortonMin = 1 << 15;
+
+// Keeps freed stream-ordered allocations in the device's default pool instead of
+// returning them to the driver at every synchronization.
+int ensure_pool(int device) {
+    static std::mutex mu;
+    static bool done[64] = {false};
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[device]) return CUBVH_OK;
+    hipMemPool_t pool;
+    hipError_t e = hipDeviceGetDefaultMemPool(&pool, device);
+    if (e != hipSuccess) return fail_hip(e, "hipDeviceGetDefaultMemPool");
+    uint64_t thr = UINT64_MAX;
+    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    if (e != hipSuccess) return fail_hip(e, "hipMemPoolSetAttribute");
+    done[device] = true;
+    return CUBVH_OK;
+}
 
 // Makes `device` current for the scope and restores the caller's device.
 struct DeviceScope {
@@ -162,7 +181,29 @@ int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* 
     q.out_f = distances;
     q.out_id = face_id;
     q.out_v = uvw;
+    hipStream_t hs = (hipStream_t)stream;
+    // Large batches are walked in Morton order (stream-ordered scratch from the
+    // device's default memory pool, released after the launch).
+    void* scratch = nullptr;
+    if (n >= kMortonMin && n < ((int64_t)1 << 31)) {
+        rc = ensure_pool(device);
+        if (rc != CUBVH_OK) return rc;
+        const size_t ob = ((size_t)n * 4 + 255) & ~(size_t)255;
+        hipError_t e = hipMallocAsync(&scratch, ob + morton_workspace_bytes(n), hs);
+        if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(query order)");
+        uint32_t* order = (uint32_t*)scratch;
+        e = morton_order(n, positions, (char*)scratch + ob, order, hs);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(scratch, hs);
+            return fail_hip(e, "query ordering");
+        }
+        q.order = order;
+    }
     hipError_t e = launch_distance(q, kmode);
+    if (scratch) {
+        hipError_t e2 = hipFreeAsync(scratch, hs);
+        if (e == hipSuccess) e = e2;
+    }
     if (e != hipSuccess) return fail_hip(e, "distance kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END

@@ -1,0 +1,126 @@
+// order.hip -- Morton ordering of point queries.
+//
+// Consecutive lanes of a wave should hold queries that are close in space, so that
+// they walk nearly the same part of the tree (the packet traversal in query.hip
+// then visits close to one lane's worth of nodes per wave, and every node or face
+// record is fetched once per wave).  The queries are not moved: a permutation is
+// computed on the device and the kernels read query order[i] and write the outputs
+// back to that position.
+//
+//   1. bounds : min/max of the query coordinates (order-preserving int atomics)
+//   2. keys   : 30-bit Morton code (10 bits per axis) of each query in those bounds
+//   3. sort   : rocPRIM radix sort (hipCUB interface) of (key, index) pairs
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "query.hpp"
+
+namespace cubvh_amd {
+namespace dev {
+
+// float <-> int mapping that preserves order under signed integer comparison
+__device__ __forceinline__ int32_t ordered(float f) {
+    int32_t i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float unordered(int32_t i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void bounds_init(int32_t* b) {
+    if (threadIdx.x < 3) b[threadIdx.x] = 0x7fffffff;
+    else if (threadIdx.x < 6) b[threadIdx.x] = (int32_t)0x80000000;
+}
+
+__global__ __launch_bounds__(256) void bounds_kernel(int64_t n, const float* __restrict__ p, int32_t* b) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float x = p[3 * i + k];
+            lo[k] = fminf(lo[k], x);
+            hi[k] = fmaxf(hi[k], x);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off, 64));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (lo[k] <= hi[k]) {  // skips waves that saw no (finite) query
+                atomicMin(b + k, ordered(lo[k]));
+                atomicMax(b + 3 + k, ordered(hi[k]));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(int64_t n, const float* __restrict__ p, const int32_t* __restrict__ b,
+                                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = unordered(b[k]), hi = unordered(b[3 + k]);
+        float ext = hi - lo;
+        float s = ext > 0.0f ? 1023.0f / ext : 0.0f;
+        float t = (p[3 * i + k] - lo) * s;
+        t = fminf(fmaxf(t, 0.0f), 1023.0f);  // NaN coordinates map to 0
+        q[k] = (uint32_t)t;
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    idx[i] = (uint32_t)i;
+}
+
+}  // namespace dev
+
+// Bytes of device workspace morton_order needs for n queries.
+size_t morton_workspace_bytes(int64_t n) {
+    size_t tmp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return al(64) + 3 * al(4 * (size_t)n) + al(tmp);
+}
+
+// Writes the Morton permutation of the n queries into order_out (n uint32) using
+// `ws` (morton_workspace_bytes(n) bytes).  n must be < 2^31.
+hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* order_out, hipStream_t s) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char* w = (char*)ws;
+    int32_t* bounds = (int32_t*)w;
+    w += al(64);
+    uint32_t* k0 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    uint32_t* k1 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    uint32_t* v0 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    size_t tmp = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::bounds_init, dim3(1), dim3(64), 0, s, bounds);
+    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, positions, bounds);
+    hipLaunchKernelGGL(dev::morton_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, positions, bounds, k0,
+                       v0);
+    e = hipcub::DeviceRadixSort::SortPairs((void*)w, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+}  // namespace cubvh_amd

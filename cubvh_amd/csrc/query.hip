@@ -334,6 +334,208 @@ __device__ f3 avg_normal(const Tree& T, f3 c, LaneStack<BLOCK, LDS_N>& st) {
     return mk3((float)sx / cnt, (float)sy / cnt, (float)sz / cnt);
 }
 
+// ---------------------------------------------------------------------------
+// Packet traversal: one wave walks the tree for its 64 queries together.
+//
+// The current node and the stack are wave-uniform; each stack entry carries the
+// 64-bit mask of lanes that still want that subtree.  Node and face records are
+// read through the constant address space with a wave-uniform address, which the
+// compiler turns into scalar loads (one fetch per wave, results in SGPRs), and
+// every lane tests them against its own query.  The stack lives in three VGPRs
+// spread across the wave's lanes (entry e in lane e, written by a lane-id select and
+// read with v_readlane), so it needs no LDS; a tree of depth D needs at most
+// 3 * D + 1 <= 64 entries.  With queries in Morton order the lanes of a wave are
+// close together and the union of their walks is close to a single walk.
+// ---------------------------------------------------------------------------
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) float* cfloat_ptr;  // constant address space
+#else
+typedef const float* cfloat_ptr;  // host pass: never executed
+#endif
+
+__device__ __forceinline__ float4 ld4(cfloat_ptr p) { return make_float4(p[0], p[1], p[2], p[3]); }
+
+struct UNode {  // wave-uniform copy of a node (scalar registers)
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 child;
+};
+
+__device__ __forceinline__ UNode load_unode(const WideNode* nodes, int32_t ref) {
+    cfloat_ptr src = (cfloat_ptr)(nodes + ref);
+    UNode u;
+    u.lx = ld4(src + 0);
+    u.hx = ld4(src + 4);
+    u.ly = ld4(src + 8);
+    u.hy = ld4(src + 12);
+    u.lz = ld4(src + 16);
+    u.hz = ld4(src + 20);
+    float4 c = ld4(src + 24);
+    u.child = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
+    return u;
+}
+
+__device__ __forceinline__ void load_utri(const float4* tris, int32_t slot, f3& a, f3& b, f3& c, int32_t& id) {
+    cfloat_ptr t = (cfloat_ptr)(tris + 3 * slot);
+    float4 t0 = ld4(t), t1 = ld4(t + 4), t2 = ld4(t + 8);
+    a = xyz(t0);
+    b = xyz(t1);
+    c = xyz(t2);
+    id = __float_as_int(t0.w);
+}
+
+__device__ __forceinline__ float box_d2(float lx, float hx, float ly, float hy, float lz, float hz, f3 p) {
+    float dx = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
+    float dy = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
+    float dz = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+struct WaveStack {
+    int32_t ref = 0, mlo = 0, mhi = 0;  // entry e held by lane e
+    int32_t sp = 0;                     // wave-uniform
+    __device__ __forceinline__ void push(int32_t r, uint64_t m) {
+        const bool here = (int32_t)(threadIdx.x & 63) == sp;
+        ref = here ? r : ref;
+        mlo = here ? (int32_t)(uint32_t)m : mlo;
+        mhi = here ? (int32_t)(uint32_t)(m >> 32) : mhi;
+        ++sp;
+    }
+    __device__ __forceinline__ bool pop(int32_t& r, uint64_t& m) {
+        if (sp == 0) return false;
+        --sp;
+        r = __builtin_amdgcn_readlane(ref, sp);
+        m = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(mlo, sp) |
+            ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, sp) << 32);
+        return true;
+    }
+};
+
+// Uniform sort of 4 (key, ref, mask) triples, ascending by key.
+__device__ __forceinline__ void ucswap(float& ka, int32_t& ra, uint64_t& ma, float& kb, int32_t& rb, uint64_t& mb) {
+    if (kb < ka) {
+        float tk = ka; ka = kb; kb = tk;
+        int32_t tr = ra; ra = rb; rb = tr;
+        uint64_t tm = ma; ma = mb; mb = tm;
+    }
+}
+
+// Nearest face for every lane of the wave (lanes with active = false only ride
+// along).  Results in the lane's best / bslot / bid, as closest_face.
+__device__ void closest_face_packet(const Tree& T, f3 p, bool active, float& best, int32_t& bslot, int32_t& bid) {
+    const int lane = (int)(threadIdx.x & 63);
+    best = MAX_DIST_SQ;
+    float bound = MAX_DIST_SQ * BOUND_SLACK;
+    bslot = -1;
+    bid = 0x7fffffff;
+    uint64_t mask = ballot(active);
+    if (mask == 0) return;
+    WaveStack st;
+    int32_t ref = 0;
+    for (;;) {
+        const bool mine = (mask >> lane) & 1;
+        if (ref >= 0) {
+            UNode nd = load_unode(T.nodes, ref);
+            float k0 = box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, p);
+            float k1 = box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, p);
+            float k2 = box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, p);
+            float k3 = box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, p);
+            uint64_t m0 = ballot(mine && k0 <= bound);
+            uint64_t m1 = ballot(mine && k1 <= bound);
+            uint64_t m2 = ballot(mine && k2 <= bound);
+            uint64_t m3 = ballot(mine && k3 <= bound);
+            // order the children by the key of their lowest interested lane
+            float r0 = m0 ? lane_value(k0, __builtin_ctzll(m0)) : INFINITY;
+            float r1 = m1 ? lane_value(k1, __builtin_ctzll(m1)) : INFINITY;
+            float r2 = m2 ? lane_value(k2, __builtin_ctzll(m2)) : INFINITY;
+            float r3 = m3 ? lane_value(k3, __builtin_ctzll(m3)) : INFINITY;
+            int32_t c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+            ucswap(r0, c0, m0, r2, c2, m2);
+            ucswap(r1, c1, m1, r3, c3, m3);
+            ucswap(r0, c0, m0, r1, c1, m1);
+            ucswap(r2, c2, m2, r3, c3, m3);
+            ucswap(r1, c1, m1, r2, c2, m2);
+            if (m0) {
+                if (m3) st.push(c3, m3);
+                if (m2) st.push(c2, m2);
+                if (m1) st.push(c1, m1);
+                ref = uniform(c0);
+                mask = m0;
+                continue;
+            }
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, c;
+                int32_t id;
+                load_utri(T.tris, first + j, a, b, c, id);
+                if (mine) {
+                    float d2 = tri_dist2(a, b, c, p);
+                    if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
+                        best = d2;
+                        bslot = first + j;
+                        bid = id;
+                        bound = best * BOUND_SLACK;
+                    }
+                }
+            }
+        }
+        if (!st.pop(ref, mask)) break;
+        ref = uniform(ref);
+    }
+}
+
+// Packet form of avg_normal: every lane averages the normals of the faces within
+// squared distance NORMAL_EPS of its own point c.
+__device__ f3 avg_normal_packet(const Tree& T, f3 c, bool active) {
+    const int lane = (int)(threadIdx.x & 63);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    float cnt = 0.0f;
+    uint64_t mask = ballot(active);
+    WaveStack st;
+    int32_t ref = 0;
+    if (mask != 0) {
+        for (;;) {
+            const bool mine = (mask >> lane) & 1;
+            if (ref >= 0) {
+                UNode nd = load_unode(T.nodes, ref);
+                uint64_t m0 = ballot(mine && box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, c) < NORMAL_EPS);
+                uint64_t m1 = ballot(mine && box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, c) < NORMAL_EPS);
+                uint64_t m2 = ballot(mine && box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, c) < NORMAL_EPS);
+                uint64_t m3 = ballot(mine && box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, c) < NORMAL_EPS);
+                if (m0) st.push(nd.child.x, m0);
+                if (m1) st.push(nd.child.y, m1);
+                if (m2) st.push(nd.child.z, m2);
+                if (m3) st.push(nd.child.w, m3);
+            } else {
+                int32_t first, count;
+                leaf_range(ref, first, count);
+                for (int j = 0; j < count; ++j) {
+                    f3 a, b, cc;
+                    int32_t id;
+                    load_utri(T.tris, first + j, a, b, cc, id);
+                    if (mine && tri_dist2(a, b, cc, c) < NORMAL_EPS) {
+                        f3 n = tri_normal(a, b, cc);
+                        sx += (double)n.x;
+                        sy += (double)n.y;
+                        sz += (double)n.z;
+                        cnt += 1.0f;
+                    }
+                }
+            }
+            if (!st.pop(ref, mask)) break;
+            ref = uniform(ref);
+        }
+    }
+    return mk3((float)sx / cnt, (float)sy / cnt, (float)sz / cnt);
+}
+
 __device__ __forceinline__ f3 load_pt(const float* __restrict__ p, int64_t i) {
     return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
@@ -346,12 +548,14 @@ __device__ __forceinline__ void store_pt(float* __restrict__ p, int64_t i, f3 v)
 // mode: 0 unsigned, 1 signed watertight, 2 signed raystab
 template <int BLOCK, int LDS_N, int MODE>
 __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float* __restrict__ positions,
+                                                         const uint32_t* __restrict__ order,
                                                          float* __restrict__ distances, int64_t* __restrict__ face_id,
                                                          float* __restrict__ uvw, Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int64_t gi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (gi >= n) return;
+    const int64_t i = order ? (int64_t)order[gi] : gi;  // query index
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
@@ -396,6 +600,48 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
     }
 }
 
+// Packet kernel for the unsigned (MODE 0) and watertight signed (MODE 1) distance.
+// Lane i of the grid handles query order[i] (or i without a permutation).
+template <int MODE>
+__global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const float* __restrict__ positions,
+                                                              const uint32_t* __restrict__ order,
+                                                              float* __restrict__ distances,
+                                                              int64_t* __restrict__ face_id, float* __restrict__ uvw,
+                                                              Tree T) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    int64_t q = 0;
+    f3 p = mk3(0.0f, 0.0f, 0.0f);
+    if (valid) {
+        q = order ? (int64_t)order[i] : i;
+        p = load_pt(positions, q);
+    }
+    float d2;
+    int32_t slot, bid;
+    closest_face_packet(T, p, valid, d2, slot, bid);
+    if (slot < 0) {  // nothing within MAX_DIST: distance 0 and face 0
+        slot = (int32_t)T.face0_slot;
+        d2 = 0.0f;
+    }
+    f3 a, b, c;
+    int32_t id = 0;
+    a = b = c = mk3(0.0f, 0.0f, 0.0f);
+    if (valid) load_tri(T.tris, slot, a, b, c, id);
+    float dist = sqrtf(d2);
+    f3 cp = mk3(0.0f, 0.0f, 0.0f);
+    if (MODE == 1 || uvw) {
+        if (valid) cp = tri_closest_pt(a, b, c, p);
+    }
+    if (MODE == 1) {
+        f3 nrm = avg_normal_packet(T, cp, valid);
+        dist = copysignf(dist, dot3(nrm, sub3(p, cp)));
+    }
+    if (!valid) return;
+    distances[q] = dist;
+    face_id[q] = (int64_t)id;
+    if (uvw) store_pt(uvw, q, tri_barycentric(a, b, c, cp));
+}
+
 template <int BLOCK, int LDS_N>
 __global__ __launch_bounds__(BLOCK) void ray_kernel(int64_t n, const float* __restrict__ rays_o,
                                                     const float* __restrict__ rays_d, float* __restrict__ positions,
@@ -429,17 +675,20 @@ constexpr int kLdsEntries = 12;
 
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
     dev::Tree T{q.nodes, q.tris, q.face0_slot};
-    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
     hipStream_t s = (hipStream_t)q.stream;
-    if (mode == 0)
-        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 0>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
-                           q.out_f, q.out_id, q.out_v, T);
-    else if (mode == 1)
-        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 1>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
-                           q.out_f, q.out_id, q.out_v, T);
-    else
-        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 2>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
-                           q.out_f, q.out_id, q.out_v, T);
+    if (mode == 0 || mode == 1) {
+        const unsigned grid = (unsigned)((q.n + 255) / 256);
+        if (mode == 0)
+            hipLaunchKernelGGL((dev::distance_packet_kernel<0>), dim3(grid), dim3(256), 0, s, q.n, q.in0, q.order,
+                               q.out_f, q.out_id, q.out_v, T);
+        else
+            hipLaunchKernelGGL((dev::distance_packet_kernel<1>), dim3(grid), dim3(256), 0, s, q.n, q.in0, q.order,
+                               q.out_f, q.out_id, q.out_v, T);
+        return hipGetLastError();
+    }
+    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 2>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                       q.order, q.out_f, q.out_id, q.out_v, T);
     return hipGetLastError();
 }
 

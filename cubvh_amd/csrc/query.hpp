@@ -12,6 +12,7 @@ struct QueryArgs {
     int64_t n = 0;
     const float* in0 = nullptr;  // positions or ray origins [n,3]
     const float* in1 = nullptr;  // ray directions [n,3]
+    const uint32_t* order = nullptr;  // optional query permutation [n] (lane i handles query order[i])
     float* out_f = nullptr;      // distances or depth [n]
     int64_t* out_id = nullptr;   // face ids [n]
     float* out_v = nullptr;      // uvw or hit positions [n,3]
@@ -28,6 +29,10 @@ int stack_capacity_for(int need);
 // mode 0 unsigned distance, 1 signed (watertight), 2 signed (raystab)
 hipError_t launch_distance(const QueryArgs& q, int mode);
 hipError_t launch_ray(const QueryArgs& q);
+
+// Morton permutation of point queries (order.hip).
+size_t morton_workspace_bytes(int64_t n);
+hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* order_out, hipStream_t s);
 
 hipError_t launch_floodfill(const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D, int32_t* labels,
                             hipStream_t stream);

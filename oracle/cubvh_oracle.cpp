@@ -219,6 +219,18 @@ inline float box_dist2(const Box& b, V3 p) {
     return dot(d, d);
 }
 
+// Squared distance to the box widened by a relative 1e-5 (plus 1e-7) per axis: a lower
+// bound that stays below every face's fp32 squared distance despite rounding; used by
+// the canonical mode of the reference-BVH walk.
+inline float box_dist2_padded(const Box& b, V3 p) {
+    auto axis = [](float lo, float hi, float x) {
+        float pad = 1e-5f * (std::fabs(lo) + std::fabs(hi)) + 1e-7f;
+        return stdmax(stdmax((lo - pad) - x, x - (hi + pad)), 0.0f);
+    };
+    V3 d = v3(axis(b.mn.x, b.mx.x, p.x), axis(b.mn.y, b.mx.y, p.y), axis(b.mn.z, b.mx.z, p.z));
+    return dot(d, d);
+}
+
 // bounding_box.cuh:151-198, returns tmin (FLT_MAX on a miss)
 inline float box_ray(const Box& b, V3 pos, V3 dir) {
     const float FMAX = std::numeric_limits<float>::max();
@@ -552,7 +564,10 @@ std::pair<int, float> ref_closest_stackless(const RefBvh& b, V3 p, float maxd2, 
     while (idx != -1) {
         const RefNode& nd = b.nodes[idx];
         if (c) c->box++;
-        if (box_dist2(nd.bb, p) > best) { idx = nd.escape_idx; continue; }
+        if (tie_mode == 0 ? box_dist2(nd.bb, p) > best : box_dist2_padded(nd.bb, p) > best * (1.0f + 1.0f / 4096.0f)) {
+            idx = nd.escape_idx;
+            continue;
+        }
         if (nd.left_idx < 0) {
             int end = -nd.right_idx - 1;
             for (int i = -nd.left_idx - 1; i < end; ++i) {
@@ -593,11 +608,16 @@ std::pair<int, float> ref_closest(const RefBvh& b, V3 p, float maxd2, Counts* c,
         } else {
             DistIdx ch[4];
             int first = nd.left_idx;
-            for (int i = 0; i < 4; ++i) ch[i] = {box_dist2(b.nodes[first + i].bb, p), first + i};
+            for (int i = 0; i < 4; ++i)
+                ch[i] = {tie_mode == 0 ? box_dist2(b.nodes[first + i].bb, p) : box_dist2_padded(b.nodes[first + i].bb, p),
+                         first + i};
             if (c) c->box += 4;
             sort4_desc(ch);
+            // canonical mode: boxes within a relative 2^-12 of the best are visited too, so
+            // that a face tied with the best is never skipped for a rounding of its box
+            const float lim = tie_mode == 0 ? best : best * (1.0f + 1.0f / 4096.0f);
             for (int i = 0; i < 4; ++i)
-                if (ch[i].dist <= best) st.push(ch[i].idx);
+                if (ch[i].dist <= lim) st.push(ch[i].idx);
         }
         if (st.over) return ref_closest_stackless(b, p, best, c, tie_mode);
     }

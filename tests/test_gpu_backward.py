@@ -1,9 +1,11 @@
 """Backward of the unsigned distance (cubvh_udf_backward) against FP64 central finite
 differences (h = 1e-4) and against an FP64 torch reference of the same closed form.
 
-Tolerances: analytic vs FD |g - g_fd| <= 2e-3 * max(|g_fd|, 1e-3 scale of the loss)
-on coordinates whose query is not near a face switch (FP64 gap to the second face
-> 1e-3, distance > 1e-3); kernel vs FP64 torch reference within 1e-5 relative."""
+Tolerances: analytic vs FD within 2e-3 relative (floor 2e-3 / N) on coordinates whose
+query is off the surface (distance > 1e-3) and whose FP64 closest point does not jump
+across the finite-difference step (|c(p+h) - c(p-h)| < 4h: the distance is smooth
+there); kernel vs FP64 torch reference within 1e-5 * |g| * (1 + 1e-2 / d), the fp32
+error of (p - c) / |p - c| growing as the distance d shrinks."""
 import numpy as np
 import pytest
 import torch
@@ -28,20 +30,24 @@ def test_point_and_vertex_gradients_vs_finite_differences(gpu, oracle):
     gv = Vt.grad.cpu().numpy().astype(np.float64)
 
     d64, f64, cp64, gap = oracle.udf_f64(v, f, pts)
-    good = (gap > 1e-3) & (d64 > 1e-3)
-    assert good.sum() > 1500
+    good = d64 > 1e-3
+    assert good.sum() > 1900
     h = 1e-4
 
     # points: dL/dp_i = 2 d_i dd_i/dp_i / n
     rng = np.random.default_rng(0)
     idx = rng.choice(np.nonzero(good)[0], 300, replace=False)
+    checked = 0
     for k in range(3):
         e = np.zeros(3)
         e[k] = h
-        dp, _, _, _ = oracle.udf_f64(v, f, pts[idx] + e)
-        dm, _, _, _ = oracle.udf_f64(v, f, pts[idx] - e)
+        dp, _, cpp, _ = oracle.udf_f64(v, f, pts[idx] + e)
+        dm, _, cpm, _ = oracle.udf_f64(v, f, pts[idx] - e)
+        smooth = np.linalg.norm(cpp - cpm, axis=1) < 4 * h
         fd = (dp * dp - dm * dm) / (2 * h) / n
-        np.testing.assert_allclose(gp[idx, k], fd, rtol=2e-3, atol=2e-3 / n)
+        np.testing.assert_allclose(gp[idx, k][smooth], fd[smooth], rtol=2e-3, atol=2e-3 / n)
+        checked += smooth.sum()
+    assert checked > 0.9 * 3 * len(idx)
 
     # vertices: FD of the whole loss over the well-conditioned queries
     sub = pts[good]
@@ -89,8 +95,13 @@ def test_gradients_vs_fp64_torch_reference(gpu, oracle):
     ref_v = torch.zeros_like(Vd)
     for k in range(3):
         ref_v.index_add_(0, tri[:, k], -(g.double() * w[:, k])[:, None] * nrm)
-    torch.testing.assert_close(P.grad.double(), ref_p, rtol=1e-5, atol=1e-6)
-    torch.testing.assert_close(Vt.grad.double(), ref_v, rtol=1e-4, atol=1e-5)
+    dd = diff.norm(dim=1, keepdim=True)
+    tol_p = 1e-5 * g.double().abs()[:, None] * (1 + 1e-2 / dd) + 1e-7
+    assert ((P.grad.double() - ref_p).abs() <= tol_p).all()
+    tol_v = torch.zeros_like(Vd)
+    for k in range(3):
+        tol_v.index_add_(0, tri[:, k], (1e-5 * (g.double().abs() * w[:, k].abs())[:, None] * (1 + 1e-2 / dd)).expand(-1, 3))
+    assert ((Vt.grad.double() - ref_v).abs() <= tol_v + 1e-6).all()
     # forward values unchanged by the autograd wrapper
     d0, f0, u0 = bvh.unsigned_distance(P.detach(), return_uvw=True)
     assert torch.equal(d0, d.detach()) and torch.equal(f0, fid) and torch.equal(u0, uvw)

@@ -103,6 +103,9 @@ def test_udf_perturbed_level6_bit_exact(gpu, oracle):
     np.testing.assert_array_equal(_np(d), od)
     np.testing.assert_array_equal(_np(fi), of)
     np.testing.assert_array_equal(_np(uvw), ou)
+    bd, bf, bu = oracle.udf_brute(v, f, p[:2000])
+    np.testing.assert_array_equal(_np(d)[:2000], bd)
+    np.testing.assert_array_equal(_np(fi)[:2000], bf)
 
 
 def test_rays_bit_exact_and_vs_fp64(gpu, oracle, ico4):
@@ -169,9 +172,10 @@ def test_sdf_raystab_bit_exact_closed_and_soup(gpu, oracle, ico4):
     np.testing.assert_array_equal(_np(fi), of)
     w = oracle.winding_f64(v, f, p)
     assert ((_np(d) < 0) == (w > 0.5))[np.abs(od) > 1e-4].all()
-    # non-watertight soup: level-4 sphere minus 20% faces plus small random triangles
+    # non-watertight soup: level-4 sphere minus 1% of its faces plus small random
+    # triangles (with 1% holes about half of the inside points still see 64 hits)
     rng = np.random.default_rng(3)
-    keep = f[np.sort(rng.permutation(len(f))[: int(0.8 * len(f))])]
+    keep = f[np.sort(rng.permutation(len(f))[: int(0.99 * len(f))])]
     c = rng.uniform(-0.6, 0.6, size=(500, 3))
     tv = (c[:, None, :] + rng.normal(0, 0.02, size=(500, 3, 3))).reshape(-1, 3)
     V2 = np.concatenate([v, tv]).astype(np.float32)

@@ -43,6 +43,19 @@ def test_refbvh_udf_equals_brute_force(oracle, ico4):
     assert (f0[clear] == fi[clear]).all()
 
 
+def test_refbvh_canonical_equals_brute_force_level6(oracle):
+    """Deeper tree with many exact ties (shared vertices of a jittered sphere)."""
+    from cubvh_amd import synthetic as S
+
+    v, f = S.jittered_sphere(6)
+    p = S.normal_points(3000, 0.4, seed=2)
+    d, fi, uvw = oracle.udf_brute(v, f, p)
+    dc, fc, uc, _ = oracle.RefBvh(v, f).udf(p, tie_mode=1)
+    np.testing.assert_array_equal(dc, d)
+    np.testing.assert_array_equal(fc, fi)
+    np.testing.assert_array_equal(uc, uvw)
+
+
 def test_refbvh_rays_equal_brute_force(oracle, ico4):
     from cubvh_amd import synthetic as S
 
@@ -81,11 +94,6 @@ def test_sdf_signs(oracle, ico4):
     assert ((dw2 < 0) == (dw < 0))[far].all()
     np.testing.assert_array_equal(dr2, dr)
     assert cw[2] > 0 and cr[2] > 0
-
-
-def _pcg32_py(advance, n):
-    spec = json.loads((GOLDEN / "pcg32_kat.json").read_text())
-    return spec
 
 
 def test_pcg32_known_answers(oracle):
