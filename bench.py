@@ -4,7 +4,7 @@ Headline (the metric's configuration): dense unsigned distance over a 256^3 grid
 [-1, 1]^3 against the perturbed level-8 icosphere (1,310,720 faces).  A step is one
 cuBVH.unsigned_distance call over all 16,777,216 grid points, inputs resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline] [--headline-only]
 
 Prints one JSON line.  `value` = M queries/s over the timed steps (wall clock
 bracketed by device synchronizations).  `roofline` prices the dominant kernel at the
@@ -12,7 +12,11 @@ algorithmic bytes of the reference's own BVH walk on the same workload
 (tests/golden/algo_bytes.json) divided by its average duration measured with HIP
 events on the launch stream.  `cpu_baseline` times the CPU restatement of the
 reference's BVH walk (oracle/) on a seeded sample of the same grid on host cores.
---extras adds fwd+bwd, watertight SDF, ray tracing, raystab and floodfill figures.
+The JSON also carries the metric's two other forms: `fwd_bwd` (jittered level-6
+sphere, 2^20 points, loss = mean(udf^2), forward + hand-written backward) and
+`batched` (512 perturbed icospheres of 1,280..327,680 faces with 4,096..262,144
+uniform points each: one autotuned, HIP-graph-replayed forest launch against 512
+per-mesh calls).  --extras adds watertight SDF, ray tracing, raystab and floodfill.
 """
 from __future__ import annotations
 
@@ -82,17 +86,17 @@ def cpu_baseline(v, f, pts, sample=1 << 17, threads=None):
     }
 
 
-def run_extras(dev, steps, warmup, out):
+def fwd_bwd_leg(dev, steps, warmup):
+    """Differentiable UDF: jittered level-6 sphere, 2^20 N(0, 0.4^2) points, loss = mean(udf^2)."""
     import torch
 
     import cubvh_amd
     from cubvh_amd import synthetic as S
 
-    ab = _algo_bytes() or {"workloads": {}}
-    # differentiable UDF: jittered level-6 sphere, 2^20 N(0, 0.4^2) points, loss = mean(udf^2)
     v6, f6 = S.jittered_sphere(6)
     bvh6 = cubvh_amd.cuBVH(v6, f6)
-    P = torch.from_numpy(S.normal_points(1 << 20, 0.4, seed=0)).to(dev)
+    n = 1 << 20
+    P = torch.from_numpy(S.normal_points(n, 0.4, seed=0)).to(dev)
     V6 = torch.from_numpy(v6).to(dev)
 
     def fwd():
@@ -104,10 +108,77 @@ def run_extras(dev, steps, warmup, out):
         d, _, _ = cubvh_amd.udf(bvh6, p, v)
         (d * d).mean().backward()
 
-    _, tf = _time_kernel(fwd, steps * 4, warmup)
-    _, tfb = _time_kernel(fwdbwd, steps * 4, warmup)
-    out["fwd_bwd_level6"] = {"queries": 1 << 20, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3,
-                             "ratio": tfb / tf, "fwd_bwd_Mq_s": (1 << 20) / tfb / 1e6}
+    _, tf = _time_kernel(fwd, steps, warmup)
+    _, tfb = _time_kernel(fwdbwd, steps, warmup)
+    return {"workload": "jittered icosphere level 6 (81,920 faces), 2^20 points N(0, 0.4^2), loss mean(udf^2)",
+            "queries": n, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf,
+            "fwd_Mq_s": n / tf / 1e6, "fwd_bwd_Mq_s": n / tfb / 1e6}
+
+
+def batched_leg(dev, steps, warmup, n_meshes=512):
+    """512 meshes with uneven query sets: per-mesh calls vs one forest launch."""
+    import torch
+
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    specs = S.forest(n_meshes, seed=0)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(1)
+    pts = [rng.uniform(-1, 1, size=(sp["n_queries"], 3)).astype(np.float32) for sp in specs]
+    for lvl in sorted({sp["level"] for sp in specs}):
+        S.icosphere_unit(lvl)  # shared topology, built once per level
+
+    def make(sp):
+        v, f, _ = S.perturbed_sphere(sp["level"], seed=sp["seed"], scale=sp["scale"])
+        return cubvh_amd.cuBVH(v, f)  # the host build runs in the library without the GIL
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        bvhs = list(ex.map(make, specs))
+    build_s = time.perf_counter() - t0
+    counts = np.array([sp["n_queries"] for sp in specs])
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    P = torch.from_numpy(np.concatenate(pts)).to(dev)
+    n = P.shape[0]
+    off = torch.from_numpy(offsets).to(dev)
+    slices = [P[offsets[m]:offsets[m + 1]] for m in range(n_meshes)]
+    dist = torch.empty(n, dtype=torch.float32, device=dev)
+    face = torch.empty(n, dtype=torch.int64, device=dev)
+
+    def plain():
+        for b, q in zip(bvhs, slices):
+            b.unsigned_distance(q)
+
+    _, t_plain = _time_kernel(plain, steps, warmup)
+    forest = cubvh_amd.cuBVHForest(bvhs)
+    tuning, table = cubvh_amd.tune_forest(forest, P, off, reps=2, use_cache=False)
+    forest.reserve(n, tuning[0], dev)
+    _, t_forest = _time_kernel(lambda: forest.unsigned_distance(P, off, False, tuning, (dist, face, None)),
+                               steps, warmup)
+    graph = cubvh_amd.ForestGraph(forest, P, off, tuning=tuning)
+    _, t_graph = _time_kernel(graph.replay, steps, warmup)
+    # the batched answers equal the per-mesh answers
+    ok = True
+    for m in range(0, n_meshes, 37):
+        d_ref, f_ref, _ = bvhs[m].unsigned_distance(slices[m])
+        ok &= bool(torch.equal(graph.dist[offsets[m]:offsets[m + 1]], d_ref))
+        ok &= bool(torch.equal(graph.face[offsets[m]:offsets[m + 1]], f_ref))
+    best = min(t_forest, t_graph)
+    return {"workload": f"{n_meshes} perturbed icospheres, {int(forest.info()['n_faces'])} faces, "
+                        f"{n} uniform points in [-1,1]^3 (4,096..262,144 per mesh)",
+            "queries": int(n), "faces": int(forest.info()["n_faces"]), "host_build_s": build_s,
+            "plain_ms": t_plain * 1e3, "forest_ms": t_forest * 1e3, "graph_ms": t_graph * 1e3,
+            "plain_Mq_s": n / t_plain / 1e6, "batched_Mq_s": n / best / 1e6, "speedup": t_plain / best,
+            "tuning": list(tuning), "equal_to_per_mesh": ok}
+
+
+def run_extras(dev, steps, warmup, out):
+    import torch
+
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
 
     # watertight SDF on the headline workload
     v8, f8, _ = S.perturbed_sphere(8, seed=0)
@@ -148,6 +219,7 @@ def main():
     ap.add_argument("--extras", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--headline-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
@@ -205,6 +277,9 @@ def main():
                                f"level 8 ({f.shape[0]} faces, seed 0)", "queries": n, "build_s": build_s},
         "roofline": roof,
     }
+    if not args.headline_only:
+        result["fwd_bwd"] = fwd_bwd_leg(dev, max(3, args.steps), args.warmup)
+        result["batched"] = batched_leg(dev, max(2, args.steps // 3), 1)
     extra = {}
     if args.extras:
         run_extras(dev, max(2, args.steps // 2), 1, extra)

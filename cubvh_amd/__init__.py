@@ -3,7 +3,9 @@
 Drop-in for the reference package's exports (cubvh/__init__.py:1-11): use
 ``import cubvh_amd as cubvh``.  The BVH query path (cuBVH, floodfill) runs in
 libcubvh_amd.so; the other reference names raise NotImplementedError.
-Additions: :func:`udf` (differentiable unsigned distance).
+Additions: :func:`udf` (differentiable unsigned distance), :class:`cuBVHForest`
+(one launch over many meshes), :class:`ForestGraph` (HIP-graph replay of a forest
+query), :func:`udf_batched`, :func:`tune_forest` (launch autotuner).
 """
 from .api import (
     cuBVH,
@@ -18,6 +20,8 @@ from .api import (
     parallel_decimate,
 )
 from .autograd import udf
+from .forest import cuBVHForest, ForestGraph, udf_batched
+from .autotune import tune_forest
 
 __all__ = [
     "cuBVH",
@@ -31,4 +35,8 @@ __all__ = [
     "decimate",
     "parallel_decimate",
     "udf",
+    "cuBVHForest",
+    "ForestGraph",
+    "udf_batched",
+    "tune_forest",
 ]

@@ -35,6 +35,14 @@ SIGNATURES = {
         [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "cubvh_floodfill": (c_int, [c_int, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
+    "cubvh_forest_create": (c_int, [c_void_p, c_int32, POINTER(c_void_p)]),
+    "cubvh_forest_destroy": (None, [c_void_p]),
+    "cubvh_forest_info": (c_int, [c_void_p, c_void_p]),
+    "cubvh_forest_reserve": (c_int, [c_void_p, c_int, c_int64, c_int32, c_void_p]),
+    "cubvh_forest_unsigned_distance": (
+        c_int,
+        [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "cubvh_udf_backward": (
         c_int,
         [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bvh.hpp"
+#include "forest.hpp"
 #include "query.hpp"
 
 using namespace cubvh_amd;
@@ -127,6 +128,86 @@ struct cubvh_bvh {
         q.stack_need = host.stack_need;
         q.stream = stream;
         return q;
+    }
+};
+
+struct cubvh_forest {
+    std::vector<WideNode> nodes;
+    std::vector<float> tris;
+    std::vector<int32_t> mesh_info;  // 4 per mesh: root, face-0 slot, n_faces, 0
+    int32_t n_meshes = 0;
+    int64_t n_faces = 0;
+    int32_t stack_need = 0;
+    std::mutex mu;
+    struct DevCopy {
+        WideNode* nodes = nullptr;
+        float4* tris = nullptr;
+        int4* mesh_info = nullptr;
+        void* ws = nullptr;
+        int64_t ws_n = -1;
+        int32_t ws_bits = -1;
+        int32_t n_cu = 0;
+    };
+    DevCopy dev[kMaxDevices];
+
+    ~cubvh_forest() {
+        for (int d = 0; d < kMaxDevices; ++d) {
+            DevCopy& c = dev[d];
+            if (!c.nodes && !c.ws) continue;
+            DeviceScope g(d);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(c.nodes);
+            (void)hipFree(c.tris);
+            (void)hipFree(c.mesh_info);
+            (void)hipFree(c.ws);
+        }
+    }
+
+    int upload(int device, hipStream_t stream) {
+        DevCopy& c = dev[device];
+        if (c.nodes) return CUBVH_OK;
+        hipError_t e = hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess) return fail_hip(e, "hipDeviceGetAttribute");
+        const size_t nb = nodes.size() * sizeof(WideNode), tb = tris.size() * sizeof(float),
+                     mb = mesh_info.size() * sizeof(int32_t);
+        e = hipMalloc(&c.nodes, nb);
+        if (e == hipSuccess) e = hipMalloc(&c.tris, tb);
+        if (e == hipSuccess) e = hipMalloc(&c.mesh_info, mb);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.nodes, nodes.data(), nb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.tris, tris.data(), tb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.mesh_info, mesh_info.data(), mb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(c.nodes);
+            (void)hipFree(c.tris);
+            (void)hipFree(c.mesh_info);
+            c.nodes = nullptr;
+            c.tris = nullptr;
+            c.mesh_info = nullptr;
+            return fail_hip(e, "forest upload");
+        }
+        return CUBVH_OK;
+    }
+
+    int reserve(int device, int64_t n, int32_t bits, hipStream_t stream) {
+        DevCopy& c = dev[device];
+        if (c.ws && c.ws_n >= n && c.ws_bits >= bits) return CUBVH_OK;
+        const int64_t nn = std::max<int64_t>(n, c.ws_n);
+        const int32_t bb = std::max<int32_t>(bits, c.ws_bits);
+        if (c.ws) {
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e == hipSuccess) e = hipFree(c.ws);
+            c.ws = nullptr;
+            if (e != hipSuccess) return fail_hip(e, "forest workspace release");
+        }
+        hipError_t e = hipMalloc(&c.ws, forest_workspace_bytes(nn, n_meshes, bb));
+        if (e != hipSuccess) {
+            c.ws = nullptr;
+            return fail_hip(e, "hipMalloc(forest workspace)");
+        }
+        c.ws_n = nn;
+        c.ws_bits = bb;
+        return CUBVH_OK;
     }
 };
 
@@ -441,6 +522,139 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
     hipError_t e = launch_udf_backward(n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
                                        grad_points, grad_vertices, (hipStream_t)stream);
     if (e != hipSuccess) return fail_hip(e, "backward kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest** out) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!bvhs || n_meshes <= 0) return fail(CUBVH_ERR_ARG, "need at least one BVH");
+    cubvh_forest* f = new cubvh_forest;
+    f->n_meshes = n_meshes;
+    f->mesh_info.assign(4 * (size_t)n_meshes, 0);
+    for (int32_t m = 0; m < n_meshes; ++m) {
+        if (!bvhs[m]) {
+            delete f;
+            return fail(CUBVH_ERR_ARG, "null BVH in the list");
+        }
+        const HostBvh& h = bvhs[m]->host;
+        const int64_t node_base = (int64_t)f->nodes.size();
+        const int64_t tri_base = f->n_faces;
+        if (tri_base + h.n_faces > MAX_LEAF_FIRST || node_base + (int64_t)h.nodes.size() > 0x7fffffff) {
+            delete f;
+            return fail(CUBVH_ERR_ARG, "forest too large for the 28-bit leaf encoding");
+        }
+        for (WideNode w : h.nodes) {
+            for (int k = 0; k < 4; ++k) {
+                int32_t r = w.child[k];
+                if (r == CHILD_EMPTY) continue;
+                if (r >= 0) {
+                    w.child[k] = (int32_t)(r + node_base);
+                } else {
+                    int64_t first = (int64_t)(((uint32_t)r & 0x7fffffffu) >> 3);
+                    w.child[k] = make_leaf_ref(first + tri_base, (r & 7) + 1);
+                }
+            }
+            f->nodes.push_back(w);
+        }
+        f->tris.insert(f->tris.end(), h.tris.begin(), h.tris.end());
+        f->mesh_info[4 * m + 0] = (int32_t)node_base;
+        f->mesh_info[4 * m + 1] = (int32_t)(tri_base + h.face0_slot);
+        f->mesh_info[4 * m + 2] = (int32_t)h.n_faces;
+        f->n_faces += h.n_faces;
+        f->stack_need = std::max(f->stack_need, h.stack_need);
+    }
+    if (stack_capacity_for(f->stack_need) < 0) {
+        delete f;
+        return fail(CUBVH_ERR_STATE, "tree too deep for the traversal stack");
+    }
+    *out = f;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+void cubvh_forest_destroy(cubvh_forest* forest) { delete forest; }
+
+int cubvh_forest_info(const cubvh_forest* forest, int64_t info[4]) {
+    if (!forest || !info) return fail(CUBVH_ERR_ARG, "null argument");
+    info[0] = forest->n_meshes;
+    info[1] = (int64_t)forest->nodes.size();
+    info[2] = forest->n_faces;
+    info[3] = forest->stack_need;
+    return CUBVH_OK;
+}
+
+int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, int32_t max_order_bits,
+                         void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!forest) return fail(CUBVH_ERR_ARG, "forest is null");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    if (max_queries < 0 || max_queries >= ((int64_t)1 << 32) - 64 * (int64_t)forest->n_meshes)
+        return fail(CUBVH_ERR_ARG, "max_queries out of range");
+    if (max_order_bits < 0 || max_order_bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
+    std::lock_guard<std::mutex> lock(forest->mu);
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    int rc = forest->upload(device, (hipStream_t)stream);
+    if (rc != CUBVH_OK) return rc;
+    return forest->reserve(device, max_queries, max_order_bits, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,
+                                   const int64_t* query_offsets, float* distances, int64_t* face_id, float* uvw,
+                                   const int32_t* tuning, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!forest) return fail(CUBVH_ERR_ARG, "forest is null");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    if (n < 0 || n >= ((int64_t)1 << 32) - 64 * (int64_t)forest->n_meshes)
+        return fail(CUBVH_ERR_ARG, "query count out of range");
+    if (n == 0) return CUBVH_OK;
+    if (!positions || !query_offsets || !distances || !face_id) return fail(CUBVH_ERR_ARG, "null query buffer");
+    const int32_t bits = tuning ? tuning[0] : 12;
+    const int32_t persistent = tuning ? tuning[1] : 1;
+    const int32_t wpc = tuning ? tuning[2] : 8;
+    if (bits < 0 || bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
+    if (wpc < 1 || wpc > 32) return fail(CUBVH_ERR_ARG, "waves per CU must be in 1..32");
+    hipStream_t hs = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(forest->mu);
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    cubvh_forest::DevCopy& c = forest->dev[device];
+    if (!c.nodes || !c.ws || c.ws_n < n || c.ws_bits < bits) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hipError_t e = hipStreamIsCapturing(hs, &cs);
+        if (e != hipSuccess) return fail_hip(e, "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(CUBVH_ERR_ARG, "forest workspace too small while capturing: call cubvh_forest_reserve first");
+        int rc = forest->upload(device, hs);
+        if (rc == CUBVH_OK) rc = forest->reserve(device, n, bits, hs);
+        if (rc != CUBVH_OK) return rc;
+    }
+    ForestWorkspace ws = carve_forest_workspace(c.ws, c.ws_n, forest->n_meshes, c.ws_bits);
+    // bins actually used by this call
+    ws.nbins = (int64_t)forest->n_meshes << bits;
+    ws.nchunks = (ws.nbins + 1023) / 1024;
+    hipError_t e = forest_order(n, positions, query_offsets, forest->n_meshes, bits, ws, hs);
+    if (e != hipSuccess) return fail_hip(e, "forest query ordering");
+    ForestArgs a;
+    a.n = n;
+    a.n_meshes = forest->n_meshes;
+    a.positions = positions;
+    a.offsets = query_offsets;
+    a.distances = distances;
+    a.face_id = face_id;
+    a.uvw = uvw;
+    a.nodes = c.nodes;
+    a.tris = c.tris;
+    a.mesh_info = c.mesh_info;
+    a.persistent = persistent;
+    a.waves_per_cu = wpc;
+    a.n_cu = c.n_cu;
+    e = launch_forest_distance(a, ws, hs);
+    if (e != hipSuccess) return fail_hip(e, "forest kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END
 }

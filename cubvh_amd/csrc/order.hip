@@ -88,6 +88,15 @@ __global__ __launch_bounds__(256) void morton_kernel(int64_t n, const float* __r
 
 }  // namespace dev
 
+hipError_t launch_bounds(int64_t n, const float* positions, int32_t* bounds, hipStream_t s) {
+    hipLaunchKernelGGL(dev::bounds_init, dim3(1), dim3(64), 0, s, bounds);
+    if (n > 0) {
+        const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
+        hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, positions, bounds);
+    }
+    return hipGetLastError();
+}
+
 // Bytes of device workspace morton_order needs for n queries.
 size_t morton_workspace_bytes(int64_t n) {
     size_t tmp = 0;
@@ -113,9 +122,8 @@ hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* o
     size_t tmp = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dev::bounds_init, dim3(1), dim3(64), 0, s, bounds);
-    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, positions, bounds);
+    e = launch_bounds(n, positions, bounds, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::morton_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, positions, bounds, k0,
                        v0);
     e = hipcub::DeviceRadixSort::SortPairs((void*)w, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);

@@ -427,7 +427,8 @@ __device__ __forceinline__ void ucswap(float& ka, int32_t& ra, uint64_t& ma, flo
 
 // Nearest face for every lane of the wave (lanes with active = false only ride
 // along).  Results in the lane's best / bslot / bid, as closest_face.
-__device__ void closest_face_packet(const Tree& T, f3 p, bool active, float& best, int32_t& bslot, int32_t& bid) {
+__device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool active, float& best, int32_t& bslot,
+                                    int32_t& bid) {
     const int lane = (int)(threadIdx.x & 63);
     best = MAX_DIST_SQ;
     float bound = MAX_DIST_SQ * BOUND_SLACK;
@@ -436,7 +437,7 @@ __device__ void closest_face_packet(const Tree& T, f3 p, bool active, float& bes
     uint64_t mask = ballot(active);
     if (mask == 0) return;
     WaveStack st;
-    int32_t ref = 0;
+    int32_t ref = uniform(root);
     for (;;) {
         const bool mine = (mask >> lane) & 1;
         if (ref >= 0) {
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
     }
     float d2;
     int32_t slot, bid;
-    closest_face_packet(T, p, valid, d2, slot, bid);
+    closest_face_packet(T, 0, p, valid, d2, slot, bid);
     if (slot < 0) {  // nothing within MAX_DIST: distance 0 and face 0
         slot = (int32_t)T.face0_slot;
         d2 = 0.0f;
@@ -640,6 +641,61 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
     distances[q] = dist;
     face_id[q] = (int64_t)id;
     if (uvw) store_pt(uvw, q, tri_barycentric(a, b, c, cp));
+}
+
+// Unsigned distance over a forest: one wave per 64-slot tile of the padded query
+// order; every tile belongs to one mesh, whose root node and face-0 slot come from
+// mesh_info.  PERSISTENT waves take tiles from an atomic counter until none is left.
+template <int PERSISTENT>
+__global__ __launch_bounds__(256) void forest_distance_kernel(const float* __restrict__ positions,
+                                                              const uint32_t* __restrict__ order,
+                                                              const int64_t* __restrict__ pbase, int32_t n_meshes,
+                                                              const int4* __restrict__ mesh_info,
+                                                              float* __restrict__ distances,
+                                                              int64_t* __restrict__ face_id, float* __restrict__ uvw,
+                                                              const WideNode* __restrict__ nodes,
+                                                              const float4* __restrict__ tris, uint32_t* counter) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t n_tiles = pbase[n_meshes] / 64;
+    int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
+    for (;;) {
+        if (PERSISTENT) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(counter, 1u);
+            tile = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)t);
+        }
+        if (tile >= n_tiles) return;
+        // mesh of this tile: largest m with pbase[m] <= 64 * tile (wave-uniform search)
+        int32_t lo = 0, hi = n_meshes - 1;
+        while (lo < hi) {
+            int32_t mid = (lo + hi + 1) >> 1;
+            if (pbase[mid] <= 64 * tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const int32_t mesh = uniform(lo);
+        const int4 mi = mesh_info[mesh];
+        const uint32_t qi = order[64 * tile + lane];
+        const bool valid = qi != 0xffffffffu;
+        f3 p = mk3(0.0f, 0.0f, 0.0f);
+        if (valid) p = load_pt(positions, qi);
+        Tree T{nodes, tris, (int64_t)mi.y};
+        float d2;
+        int32_t slot, bid;
+        closest_face_packet(T, uniform(mi.x), p, valid, d2, slot, bid);
+        if (valid) {
+            if (slot < 0) {
+                slot = mi.y;
+                d2 = 0.0f;
+            }
+            f3 a, b, c;
+            int32_t id;
+            load_tri(tris, slot, a, b, c, id);
+            distances[qi] = sqrtf(d2);
+            face_id[qi] = (int64_t)id;
+            if (uvw) store_pt(uvw, qi, tri_barycentric(a, b, c, tri_closest_pt(a, b, c, p)));
+        }
+        if (!PERSISTENT) return;
+    }
 }
 
 template <int BLOCK, int LDS_N>
@@ -693,6 +749,29 @@ hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
 }
 
 }  // namespace
+
+}  // namespace cubvh_amd
+
+#include "forest.hpp"
+
+namespace cubvh_amd {
+
+hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    if (a.persistent) {
+        const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
+        hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
+                           ws.counter);
+    } else {
+        const int64_t max_tiles = ws.npad_max / 64;
+        const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
+        hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
+                           ws.counter);
+    }
+    return hipGetLastError();
+}
 
 int stack_capacity_for(int need) { return need <= dev::STACK_TOTAL ? dev::STACK_TOTAL : -1; }
 

@@ -133,6 +133,42 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
                        const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
                        void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched queries over many meshes.  No counterpart in the reference (which takes
+ * one mesh per cuBVH and one launch per query call); for callers with many meshes
+ * and uneven query sets per mesh.
+ * ------------------------------------------------------------------------- */
+typedef struct cubvh_forest cubvh_forest;
+
+/* Concatenates the trees of n_meshes BVHs (host only; the BVHs stay owned by the
+ * caller and may be destroyed afterwards).  Face ids reported by the forest are the
+ * per-mesh original face indices. */
+int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest** out);
+void cubvh_forest_destroy(cubvh_forest* forest);
+
+/* info[0] = n_meshes, info[1] = total nodes, info[2] = total faces, info[3] = stack
+ * entries needed. */
+int cubvh_forest_info(const cubvh_forest* forest, int64_t info[4]);
+
+/* Uploads the forest to `device` and allocates the query workspace for up to
+ * max_queries queries and Morton-prefix bins of up to max_order_bits bits.  After
+ * this call, cubvh_forest_unsigned_distance within those limits allocates nothing
+ * and issues only kernel launches (safe to capture in a hipGraph).  Synchronous. */
+int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, int32_t max_order_bits,
+                         void* stream);
+
+/* Unsigned distance of n queries, positions fp32 [n,3] concatenated mesh by mesh;
+ * query_offsets: int64 [n_meshes + 1] ON THE DEVICE, queries of mesh m are
+ * [query_offsets[m], query_offsets[m+1]).  Outputs as cubvh_unsigned_distance.
+ * tuning (nullable = {12, 1, 8}): [0] Morton-prefix bits used to order the queries
+ * inside each mesh (0..16, 0 keeps input order), [1] 1 = persistent grid taking
+ * 64-query tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of
+ * the persistent grid (1..32).  Grows the workspace when the reserved one is too
+ * small, which is refused while the stream is being captured. */
+int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,
+                                   const int64_t* query_offsets, float* distances, int64_t* face_id, float* uvw,
+                                   const int32_t* tuning, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

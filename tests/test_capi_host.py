@@ -199,3 +199,20 @@ def test_floodfill_requires_bool_grid():
 
     with pytest.raises(TypeError, match="bool"):
         cubvh_amd.floodfill(torch.zeros(4, 4, 4, dtype=torch.uint8))
+
+
+def test_forest_host_concatenation():
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    bs = [cubvh_amd.cuBVH(*S.icosphere(l)) for l in (2, 3, 1)]
+    f = cubvh_amd.cuBVHForest(bs)
+    info = f.info()
+    assert info["n_meshes"] == 3
+    assert info["n_faces"] == sum(b.info()["n_faces"] for b in bs)
+    assert info["n_nodes"] == sum(b.info()["n_nodes"] for b in bs)
+    assert info["stack_need"] == max(b.info()["stack_need"] for b in bs)
+    vcat, fcat = f.concatenated_mesh("cpu")
+    assert vcat.shape[0] == sum(b._vertices.shape[0] for b in bs)
+    assert fcat.shape[0] == info["n_faces"] and int(fcat.max()) == vcat.shape[0] - 1
+    assert f.face_base().tolist() == [0, 320, 1600]
