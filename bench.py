@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device-to-device tensor copy first (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
@@ -245,6 +247,14 @@ def main():
     import ctypes
 
     stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         cubvh_amd._lib.check(lib.cubvh_unsigned_distance(bvh._h, dev.index, n, P.data_ptr(), dist.data_ptr(),
