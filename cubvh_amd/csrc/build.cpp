@@ -317,7 +317,47 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
         o[8] = t[6]; o[9] = t[7]; o[10] = t[8];
         if (ids[f] == 0) out.face0_slot = s;
     }
+    compute_face_bounds(out);
     return "";
+}
+
+void compute_face_bounds(HostBvh& bvh) {
+    const int64_t m = bvh.n_faces;
+    bvh.tbnd.assign(8 * (size_t)m, 0.0f);
+    for (int64_t s = 0; s < m; ++s) {
+        const float* t = &bvh.tris[12 * s];
+        double v[3][3] = {{t[0], t[1], t[2]}, {t[4], t[5], t[6]}, {t[8], t[9], t[10]}};
+        float cf[3];
+        for (int k = 0; k < 3; ++k) cf[k] = (float)((v[0][k] + v[1][k] + v[2][k]) / 3.0);
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = v[1][k] - v[0][k];
+            e2[k] = v[2][k] - v[0][k];
+        }
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        float nf[3] = {0.0f, 0.0f, 0.0f};
+        if (nl > 0.0 && std::isfinite(nl))
+            for (int k = 0; k < 3; ++k) nf[k] = (float)(n[k] / nl);
+        // radius (in-plane distance) and thickness (off-plane distance) with respect to
+        // the rounded centre and normal, both from the exact vertex coordinates
+        double r = 0.0, th = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double d[3] = {v[i][0] - cf[0], v[i][1] - cf[1], v[i][2] - cf[2]};
+            double h = d[0] * nf[0] + d[1] * nf[1] + d[2] * nf[2];
+            double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            r = std::max(r, std::sqrt(std::max(dd - h * h, 0.0)));
+            th = std::max(th, std::fabs(h));
+        }
+        r = r * (1.0 + 1e-6) + 1e-30;
+        th = th * (1.0 + 1e-6) + 1e-30;
+        float rf = (float)r, tf = (float)th;
+        if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
+        if ((double)tf < th) tf = std::nextafter(tf, INFINITY);
+        float* o = &bvh.tbnd[8 * s];
+        o[0] = cf[0]; o[1] = cf[1]; o[2] = cf[2]; o[3] = rf;
+        o[4] = nf[0]; o[5] = nf[1]; o[6] = nf[2]; o[7] = tf;
+    }
 }
 
 std::string validate_bvh(HostBvh& bvh) {

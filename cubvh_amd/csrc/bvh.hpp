@@ -10,6 +10,11 @@
 //           tris[3t+0] = (a.x, a.y, a.z, bits of the original face index),
 //           tris[3t+1] = (b.x, b.y, b.z, 0), tris[3t+2] = (c.x, c.y, c.z, 0).
 //           A leaf is a contiguous run of at most 8 triangles.
+//   tbnd  : float4[2 * n_faces] in leaf order, a bounding disk per face:
+//           tbnd[2t+0] = (centre, radius), tbnd[2t+1] = (unit normal, thickness).
+//           Every point of the face lies within `radius` of the centre in the plane
+//           through the centre with that normal, and within `thickness` of that plane.
+//           It gives a cheap lower bound on the squared distance to the face.
 //
 // Child references (WideNode::child[k]):
 //   ref >= 0                  internal node index
@@ -46,6 +51,7 @@ static_assert(sizeof(WideNode) == 128, "wide node must be one 128-byte line");
 struct HostBvh {
     std::vector<WideNode> nodes;
     std::vector<float> tris;  // 12 floats per triangle, leaf order, see header comment
+    std::vector<float> tbnd;  // 8 floats per triangle, leaf order, see header comment
     int64_t n_faces = 0;
     int32_t depth = 0;        // root = 1
     int32_t stack_need = 0;   // entries a nearest-first traversal may hold
@@ -56,6 +62,9 @@ struct HostBvh {
 // Builds the tree over triangles given as 9 floats each (a, b, c) with original ids.
 // Returns an empty string on success, else an error message.
 std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out);
+
+// Fills bvh.tbnd from bvh.tris (bounding disk of every face, rounded outward).
+void compute_face_bounds(HostBvh& bvh);
 
 // Recomputes depth / stack_need / face0_slot of an imported tree and validates every
 // reference.  Returns an empty string on success.

@@ -74,6 +74,7 @@ struct cubvh_bvh {
     struct DevCopy {
         WideNode* nodes = nullptr;
         float4* tris = nullptr;
+        float4* tbnd = nullptr;
     };
     DevCopy dev[kMaxDevices];
 
@@ -84,6 +85,7 @@ struct cubvh_bvh {
             (void)hipDeviceSynchronize();
             (void)hipFree(dev[d].nodes);
             (void)hipFree(dev[d].tris);
+            (void)hipFree(dev[d].tbnd);
         }
     }
 
@@ -97,25 +99,26 @@ struct cubvh_bvh {
         if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
         const size_t nb = host.nodes.size() * sizeof(WideNode);
         const size_t tb = host.tris.size() * sizeof(float);
+        const size_t bb = host.tbnd.size() * sizeof(float);
         WideNode* n = nullptr;
         float4* t = nullptr;
+        float4* b = nullptr;
         hipError_t e = hipMalloc(&n, nb);
-        if (e != hipSuccess) return fail_hip(e, "hipMalloc(nodes)");
-        e = hipMalloc(&t, tb);
-        if (e != hipSuccess) {
-            (void)hipFree(n);
-            return fail_hip(e, "hipMalloc(triangles)");
-        }
-        e = hipMemcpyAsync(n, host.nodes.data(), nb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMalloc(&t, tb);
+        if (e == hipSuccess) e = hipMalloc(&b, bb);
+        if (e == hipSuccess) e = hipMemcpyAsync(n, host.nodes.data(), nb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(t, host.tris.data(), tb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b, host.tbnd.data(), bb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             (void)hipFree(n);
             (void)hipFree(t);
+            (void)hipFree(b);
             return fail_hip(e, "upload");
         }
         c.nodes = n;
         c.tris = t;
+        c.tbnd = b;
         return CUBVH_OK;
     }
 
@@ -124,6 +127,7 @@ struct cubvh_bvh {
         q.n = n;
         q.nodes = dev[device].nodes;
         q.tris = dev[device].tris;
+        q.tbnd = dev[device].tbnd;
         q.face0_slot = host.face0_slot;
         q.stack_need = host.stack_need;
         q.stream = stream;
@@ -134,6 +138,7 @@ struct cubvh_bvh {
 struct cubvh_forest {
     std::vector<WideNode> nodes;
     std::vector<float> tris;
+    std::vector<float> tbnd;
     std::vector<int32_t> mesh_info;  // 4 per mesh: root, face-0 slot, n_faces, 0
     int32_t n_meshes = 0;
     int64_t n_faces = 0;
@@ -142,6 +147,7 @@ struct cubvh_forest {
     struct DevCopy {
         WideNode* nodes = nullptr;
         float4* tris = nullptr;
+        float4* tbnd = nullptr;
         int4* mesh_info = nullptr;
         void* ws = nullptr;
         int64_t ws_n = -1;
@@ -158,6 +164,7 @@ struct cubvh_forest {
             (void)hipDeviceSynchronize();
             (void)hipFree(c.nodes);
             (void)hipFree(c.tris);
+            (void)hipFree(c.tbnd);
             (void)hipFree(c.mesh_info);
             (void)hipFree(c.ws);
         }
@@ -169,20 +176,24 @@ struct cubvh_forest {
         hipError_t e = hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device);
         if (e != hipSuccess) return fail_hip(e, "hipDeviceGetAttribute");
         const size_t nb = nodes.size() * sizeof(WideNode), tb = tris.size() * sizeof(float),
-                     mb = mesh_info.size() * sizeof(int32_t);
+                     bb = tbnd.size() * sizeof(float), mb = mesh_info.size() * sizeof(int32_t);
         e = hipMalloc(&c.nodes, nb);
         if (e == hipSuccess) e = hipMalloc(&c.tris, tb);
+        if (e == hipSuccess) e = hipMalloc(&c.tbnd, bb);
         if (e == hipSuccess) e = hipMalloc(&c.mesh_info, mb);
         if (e == hipSuccess) e = hipMemcpyAsync(c.nodes, nodes.data(), nb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c.tris, tris.data(), tb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.tbnd, tbnd.data(), bb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c.mesh_info, mesh_info.data(), mb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             (void)hipFree(c.nodes);
             (void)hipFree(c.tris);
+            (void)hipFree(c.tbnd);
             (void)hipFree(c.mesh_info);
             c.nodes = nullptr;
             c.tris = nullptr;
+            c.tbnd = nullptr;
             c.mesh_info = nullptr;
             return fail_hip(e, "forest upload");
         }
@@ -388,6 +399,7 @@ int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* trian
         delete b;
         return fail(CUBVH_ERR_STATE, "invalid state: " + err);
     }
+    compute_face_bounds(h);
     return finish_bvh(b, out);
     CUBVH_GUARD_END
 }
@@ -560,6 +572,7 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
             f->nodes.push_back(w);
         }
         f->tris.insert(f->tris.end(), h.tris.begin(), h.tris.end());
+        f->tbnd.insert(f->tbnd.end(), h.tbnd.begin(), h.tbnd.end());
         f->mesh_info[4 * m + 0] = (int32_t)node_base;
         f->mesh_info[4 * m + 1] = (int32_t)(tri_base + h.face0_slot);
         f->mesh_info[4 * m + 2] = (int32_t)h.n_faces;
@@ -649,6 +662,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.uvw = uvw;
     a.nodes = c.nodes;
     a.tris = c.tris;
+    a.tbnd = c.tbnd;
     a.mesh_info = c.mesh_info;
     a.persistent = persistent;
     a.waves_per_cu = wpc;

@@ -35,8 +35,27 @@ constexpr float BOUND_SLACK = 1.0f + 1.0f / 4096.0f;
 struct Tree {
     const WideNode* __restrict__ nodes;
     const float4* __restrict__ tris;
+    const float4* __restrict__ tbnd;  // per-face bounding disks
     int64_t face0_slot;
 };
+
+// True when face `slot` cannot come within squared distance `bound` of p, judged from
+// its bounding disk (centre c, radius r, unit normal n, thickness t): every point x
+// of the face has |p - x|^2 >= max(|h| - t, 0)^2 + max(rho - r, 0)^2 with
+// h = n.(p - c) and rho the in-plane distance from c.  The margin 2^-15 (|p-c|^2 + r^2)
+// exceeds the fp32 rounding of this bound and of the exact face test, so a rejected
+// face could not have produced a squared distance <= bound (nor tied the best).
+__device__ __forceinline__ bool disk_rejects(f3 p, float4 cb, float4 nb, float bound) {
+    f3 d = mk3(p.x - cb.x, p.y - cb.y, p.z - cb.z);
+    float dd = dot3(d, d);
+    float h = dot3(mk3(nb.x, nb.y, nb.z), d);
+    float ah = fmaxf(fabsf(h) - nb.w, 0.0f);
+    float rho = __builtin_amdgcn_sqrtf(fmaxf(dd - h * h, 0.0f));
+    float t = fmaxf(rho - cb.w, 0.0f);
+    float lb = ah * ah + t * t;
+    float margin = 3.0517578125e-05f * (dd + cb.w * cb.w);
+    return lb - margin > bound;
+}
 
 __device__ __forceinline__ void load_tri(const float4* __restrict__ tris, int32_t slot, f3& a, f3& b, f3& c,
                                          int32_t& id) {
@@ -152,6 +171,7 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
             int32_t first, count;
             leaf_range(ref, first, count);
             for (int j = 0; j < count; ++j) {
+                if (disk_rejects(p, T.tbnd[2 * (first + j)], T.tbnd[2 * (first + j) + 1], bound)) continue;
                 f3 a, b, c;
                 int32_t id;
                 load_tri(T.tris, first + j, a, b, c, id);
@@ -473,10 +493,13 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
             int32_t first, count;
             leaf_range(ref, first, count);
             for (int j = 0; j < count; ++j) {
+                cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
+                const bool cand = mine && !disk_rejects(p, ld4(bp), ld4(bp + 4), bound);
+                if (ballot(cand) == 0) continue;  // no lane can reach this face
                 f3 a, b, c;
                 int32_t id;
                 load_utri(T.tris, first + j, a, b, c, id);
-                if (mine) {
+                if (cand) {
                     float d2 = tri_dist2(a, b, c, p);
                     if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
                         best = d2;
@@ -654,7 +677,8 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
                                                               const WideNode* __restrict__ nodes,
-                                                              const float4* __restrict__ tris, uint32_t* counter) {
+                                                              const float4* __restrict__ tris,
+                                                              const float4* __restrict__ tbnd, uint32_t* counter) {
     const int lane = (int)(threadIdx.x & 63);
     const int64_t n_tiles = pbase[n_meshes] / 64;
     int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -678,7 +702,7 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
         const bool valid = qi != 0xffffffffu;
         f3 p = mk3(0.0f, 0.0f, 0.0f);
         if (valid) p = load_pt(positions, qi);
-        Tree T{nodes, tris, (int64_t)mi.y};
+        Tree T{nodes, tris, tbnd, (int64_t)mi.y};
         float d2;
         int32_t slot, bid;
         closest_face_packet(T, uniform(mi.x), p, valid, d2, slot, bid);
@@ -730,7 +754,7 @@ constexpr int kBlock = 128;
 constexpr int kLdsEntries = 12;
 
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
-    dev::Tree T{q.nodes, q.tris, q.face0_slot};
+    dev::Tree T{q.nodes, q.tris, q.tbnd, q.face0_slot};
     hipStream_t s = (hipStream_t)q.stream;
     if (mode == 0 || mode == 1) {
         const unsigned grid = (unsigned)((q.n + 255) / 256);
@@ -762,13 +786,13 @@ hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws
         const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
                            ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           ws.counter);
+                           a.tbnd, ws.counter);
     } else {
         const int64_t max_tiles = ws.npad_max / 64;
         const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
                            ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           ws.counter);
+                           a.tbnd, ws.counter);
     }
     return hipGetLastError();
 }
@@ -784,7 +808,7 @@ hipError_t launch_distance(const QueryArgs& q, int mode) {
 hipError_t launch_ray(const QueryArgs& q) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
-    dev::Tree T{q.nodes, q.tris, q.face0_slot};
+    dev::Tree T{q.nodes, q.tris, q.tbnd, q.face0_slot};
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((dev::ray_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, (hipStream_t)q.stream, q.n,
                        q.in0, q.in1, q.out_v, q.out_id, q.out_f, T);

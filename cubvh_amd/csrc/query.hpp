@@ -18,6 +18,7 @@ struct QueryArgs {
     float* out_v = nullptr;      // uvw or hit positions [n,3]
     const WideNode* nodes = nullptr;
     const float4* tris = nullptr;
+    const float4* tbnd = nullptr;  // per-face bounding disks (bvh.hpp)
     int64_t face0_slot = 0;
     int32_t stack_need = 0;
     void* stream = nullptr;
