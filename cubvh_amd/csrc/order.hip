@@ -31,9 +31,13 @@ __global__ void bounds_init(int32_t* b) {
     else if (threadIdx.x < 6) b[threadIdx.x] = (int32_t)0x80000000;
 }
 
-__global__ __launch_bounds__(256) void bounds_kernel(int64_t n, const float* __restrict__ p, int32_t* b) {
+// Bounds of every `stride`-th query (the ordering only needs approximate bounds:
+// queries outside them are clamped to the boundary cells).
+__global__ __launch_bounds__(256) void bounds_kernel(int64_t n, int64_t stride, const float* __restrict__ p,
+                                                     int32_t* b) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * stride; i < n;
+         i += (int64_t)gridDim.x * blockDim.x * stride) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             float x = p[3 * i + k];
@@ -91,8 +95,10 @@ __global__ __launch_bounds__(256) void morton_kernel(int64_t n, const float* __r
 hipError_t launch_bounds(int64_t n, const float* positions, int32_t* bounds, hipStream_t s) {
     hipLaunchKernelGGL(dev::bounds_init, dim3(1), dim3(64), 0, s, bounds);
     if (n > 0) {
-        const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
-        hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, positions, bounds);
+        const int64_t stride = std::max<int64_t>(1, n >> 20);
+        const int64_t m = (n + stride - 1) / stride;
+        const unsigned nb = (unsigned)std::min<int64_t>((m + 255) / 256, 1024);
+        hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, stride, positions, bounds);
     }
     return hipGetLastError();
 }
