@@ -42,6 +42,18 @@ def _algo_bytes():
     return json.loads(p.read_text())
 
 
+def _traffic_per_launch():
+    """HBM traffic of the headline kernel from the committed counter summary."""
+    p = ROOT / "profiles" / "udf_level8_grid256_traffic.md"
+    try:
+        for line in p.read_text().splitlines():
+            if "traffic_bytes_per_launch:" in line:
+                return float(line.split(":")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def _time_kernel(fn, steps, warmup):
     """Wall time (synchronized) and HIP-event time per call on the current stream."""
     import torch
@@ -268,7 +280,7 @@ def main():
         bpq = ab["workloads"]["udf_level8_grid256"]["bytes_per_query"]
         achieved = bpq * n / ev / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic_per_launch(),
                 "bytes_per_query": bpq, "kernel_ms": ev * 1e3}
     result = {
         "metric": "M UDF queries/s @1.3M tris 256^3 (plain vs fwd+bwd vs batched), % HBM roofline",

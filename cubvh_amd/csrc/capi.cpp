@@ -32,8 +32,9 @@ int fail_hip(hipError_t e, const char* what) {
 }
 
 constexpr int kMaxDevices = 64;
-// Query batches at least this large are sorted into Morton order before the walk.
-constexpr int64_t kMortonMin = 1 << 15;
+// Point-query batches at least this large are sorted into Morton order and walked as
+// wave packets; smaller ones are walked one lane per query in input order.
+constexpr int64_t kMortonMin = 1 << 11;
 
 // Keeps freed stream-ordered allocations in the device's default pool instead of
 // returning them to the driver at every synchronization.
@@ -291,6 +292,7 @@ int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* 
         }
         q.order = order;
     }
+    q.packet = scratch != nullptr;
     hipError_t e = launch_distance(q, kmode);
     if (scratch) {
         hipError_t e2 = hipFreeAsync(scratch, hs);

@@ -756,7 +756,7 @@ constexpr int kLdsEntries = 12;
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
     dev::Tree T{q.nodes, q.tris, q.tbnd, q.face0_slot};
     hipStream_t s = (hipStream_t)q.stream;
-    if (mode == 0 || mode == 1) {
+    if ((mode == 0 || mode == 1) && q.packet) {
         const unsigned grid = (unsigned)((q.n + 255) / 256);
         if (mode == 0)
             hipLaunchKernelGGL((dev::distance_packet_kernel<0>), dim3(grid), dim3(256), 0, s, q.n, q.in0, q.order,
@@ -767,8 +767,15 @@ hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
         return hipGetLastError();
     }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 2>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
-                       q.order, q.out_f, q.out_id, q.out_v, T);
+    if (mode == 0)
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 0>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.order, q.out_f, q.out_id, q.out_v, T);
+    else if (mode == 1)
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 1>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.order, q.out_f, q.out_id, q.out_v, T);
+    else
+        hipLaunchKernelGGL((dev::distance_kernel<kBlock, kLdsEntries, 2>), dim3(grid), dim3(kBlock), 0, s, q.n, q.in0,
+                           q.order, q.out_f, q.out_id, q.out_v, T);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ struct QueryArgs {
     const float4* tbnd = nullptr;  // per-face bounding disks (bvh.hpp)
     int64_t face0_slot = 0;
     int32_t stack_need = 0;
+    bool packet = true;  // walk as wave packets (queries in Morton order) or one lane per query
     void* stream = nullptr;
 };
 
