@@ -436,13 +436,22 @@ struct WaveStack {
     }
 };
 
-// Uniform sort of 4 (key, ref, mask) triples, ascending by key.
-__device__ __forceinline__ void ucswap(float& ka, int32_t& ra, uint64_t& ma, float& kb, int32_t& rb, uint64_t& mb) {
+// Uniform sort of 4 (key, ref, mask) triples, ascending by key.  Keys are
+// non-negative floats (or +inf) passed as their bit patterns, which order the same
+// way as unsigned integers, so the network runs on the scalar unit.
+__device__ __forceinline__ void ucswap(uint32_t& ka, int32_t& ra, uint64_t& ma, uint32_t& kb, int32_t& rb,
+                                       uint64_t& mb) {
     if (kb < ka) {
-        float tk = ka; ka = kb; kb = tk;
+        uint32_t tk = ka; ka = kb; kb = tk;
         int32_t tr = ra; ra = rb; rb = tr;
         uint64_t tm = ma; ma = mb; mb = tm;
     }
+}
+
+// Key of the lowest lane in m (+inf bits when m is empty).
+__device__ __forceinline__ uint32_t rep_key(float k, uint64_t m) {
+    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(k), (int)__builtin_ctzll(m | 1ull));
+    return m ? v : 0x7f800000u;
 }
 
 // Nearest face for every lane of the wave (lanes with active = false only ride
@@ -459,29 +468,25 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
     WaveStack st;
     int32_t ref = uniform(root);
     for (;;) {
-        const bool mine = (mask >> lane) & 1;
         if (ref >= 0) {
             UNode nd = load_unode(T.nodes, ref);
-            float k0 = box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, p);
-            float k1 = box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, p);
-            float k2 = box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, p);
-            float k3 = box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, p);
-            uint64_t m0 = ballot(mine && k0 <= bound);
-            uint64_t m1 = ballot(mine && k1 <= bound);
-            uint64_t m2 = ballot(mine && k2 <= bound);
-            uint64_t m3 = ballot(mine && k3 <= bound);
-            // order the children by the key of their lowest interested lane
-            float r0 = m0 ? lane_value(k0, __builtin_ctzll(m0)) : INFINITY;
-            float r1 = m1 ? lane_value(k1, __builtin_ctzll(m1)) : INFINITY;
-            float r2 = m2 ? lane_value(k2, __builtin_ctzll(m2)) : INFINITY;
-            float r3 = m3 ? lane_value(k3, __builtin_ctzll(m3)) : INFINITY;
-            int32_t c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
-            ucswap(r0, c0, m0, r2, c2, m2);
-            ucswap(r1, c1, m1, r3, c3, m3);
-            ucswap(r0, c0, m0, r1, c1, m1);
-            ucswap(r2, c2, m2, r3, c3, m3);
-            ucswap(r1, c1, m1, r2, c2, m2);
-            if (m0) {
+            // every lane computes; the uniform lane mask selects the interested ones
+            uint64_t m0 = ballot(box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, p) <= bound) & mask;
+            uint64_t m1 = ballot(box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, p) <= bound) & mask;
+            uint64_t m2 = ballot(box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, p) <= bound) & mask;
+            uint64_t m3 = ballot(box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, p) <= bound) & mask;
+            if ((m0 | m1 | m2 | m3) != 0) {
+                // order the children by the key of their lowest interested lane
+                uint32_t r0 = rep_key(box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, p), m0);
+                uint32_t r1 = rep_key(box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, p), m1);
+                uint32_t r2 = rep_key(box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, p), m2);
+                uint32_t r3 = rep_key(box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, p), m3);
+                int32_t c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+                ucswap(r0, c0, m0, r2, c2, m2);
+                ucswap(r1, c1, m1, r3, c3, m3);
+                ucswap(r0, c0, m0, r1, c1, m1);
+                ucswap(r2, c2, m2, r3, c3, m3);
+                ucswap(r1, c1, m1, r2, c2, m2);
                 if (m3) st.push(c3, m3);
                 if (m2) st.push(c2, m2);
                 if (m1) st.push(c1, m1);
@@ -494,12 +499,12 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
             leaf_range(ref, first, count);
             for (int j = 0; j < count; ++j) {
                 cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
-                const bool cand = mine && !disk_rejects(p, ld4(bp), ld4(bp + 4), bound);
-                if (ballot(cand) == 0) continue;  // no lane can reach this face
+                const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & mask;
+                if (cm == 0) continue;  // no lane can reach this face
                 f3 a, b, c;
                 int32_t id;
                 load_utri(T.tris, first + j, a, b, c, id);
-                if (cand) {
+                if ((cm >> lane) & 1) {
                     float d2 = tri_dist2(a, b, c, p);
                     if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
                         best = d2;
