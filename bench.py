@@ -232,6 +232,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--leaf", type=int, default=0, help="leaf_size argument of cuBVH (0 = library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device tensor copy first (reference for HBM counters)")
     args = ap.parse_args()
@@ -248,7 +249,7 @@ def main():
     dev = torch.device("cuda:0")
     v, f, meta = S.perturbed_sphere(8, seed=0)
     t0 = time.perf_counter()
-    bvh = cubvh_amd.cuBVH(v, f)
+    bvh = cubvh_amd.cuBVH(v, f, leaf_size=args.leaf)
     build_s = time.perf_counter() - t0
     pts = S.grid_points(args.res)
     P = torch.from_numpy(pts).to(dev)
@@ -296,7 +297,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"dense unsigned_distance, {args.res}^3 grid on [-1,1]^3, perturbed icosphere "
-                               f"level 8 ({f.shape[0]} faces, seed 0)", "queries": n, "build_s": build_s},
+                               f"level 8 ({f.shape[0]} faces, seed 0)", "queries": n, "build_s": build_s, "tree": bvh.info()},
         "roofline": roof,
     }
     if not args.headline_only:

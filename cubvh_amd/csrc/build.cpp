@@ -2,14 +2,15 @@
 //
 // Replaces the reference's single-threaded build (src/bvh.cu:515-620: 4-ary median
 // split on the max-variance axis, leaves <= 8).  This builder:
-//   1. splits a binary tree top-down at the object median of the axis with the
-//      largest centroid extent (std::nth_element), down to leaves of <= leaf_size
-//      triangles, building independent subtrees on worker threads;
+//   1. splits a binary tree top-down, by default with a binned surface-area
+//      heuristic (16 centroid bins per axis), optionally at the object median of the
+//      axis with the largest centroid extent (std::nth_element), down to leaves of
+//      <= leaf_size triangles, building independent subtrees on worker threads;
 //   2. collapses it into 4-wide nodes by repeatedly opening the child with the
 //      largest surface area, so every wide node has up to 4 children;
 //   3. writes nodes in depth-first pre-order and triangles in leaf order.
-// Balanced splits keep the tree shallow (depth ~ log4(M / leaf_size)), which
-// bounds the per-lane traversal stack that the kernels keep in LDS.
+// Trees deeper than the traversal stack allows (3 * depth + 1 > 64 entries) are
+// refused by the caller; median splits always stay far below that.
 //
 // Child boxes are inflated outward by a few ulps of the scene scale so that the
 // fp32 box tests stay conservative with respect to the fp32 triangle tests: a box
@@ -89,8 +90,78 @@ struct Builder {
         return me;
     }
 
+    int split_mode = 0;  // 0 object median, 1 binned SAH
+
+    int64_t split(int64_t b, int64_t e) { return split_mode == 1 ? split_sah(b, e) : split_median(b, e); }
+
+    // Binned surface-area heuristic over 16 centroid bins per axis; falls back to the
+    // object median when every candidate leaves one side empty.
+    int64_t split_sah(int64_t b, int64_t e) {
+        constexpr int NB = 16;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = b; i < e; ++i) {
+            const float* c = &cen[3 * idx[i]];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], c[k]);
+                hi[k] = std::max(hi[k], c[k]);
+            }
+        }
+        float best_cost = INFINITY;
+        int best_axis = -1, best_bin = -1;
+        for (int axis = 0; axis < 3; ++axis) {
+            const float ext = hi[axis] - lo[axis];
+            if (!(ext > 0.0f)) continue;
+            Aabb bb[NB];
+            int64_t cnt[NB] = {0};
+            const float sc = NB / ext;
+            for (int64_t i = b; i < e; ++i) {
+                int k = (int)((cen[3 * idx[i] + axis] - lo[axis]) * sc);
+                k = std::min(std::max(k, 0), NB - 1);
+                cnt[k]++;
+                bb[k].grow(tbox[idx[i]]);
+            }
+            Aabb right[NB];
+            int64_t rc[NB];
+            Aabb acc;
+            int64_t ac = 0;
+            for (int k = NB - 1; k >= 1; --k) {
+                acc.grow(bb[k]);
+                ac += cnt[k];
+                right[k] = acc;
+                rc[k] = ac;
+            }
+            Aabb left;
+            int64_t lc = 0;
+            for (int k = 0; k < NB - 1; ++k) {
+                left.grow(bb[k]);
+                lc += cnt[k];
+                if (lc == 0 || rc[k + 1] == 0) continue;
+                float cost = left.area() * (float)lc + right[k + 1].area() * (float)rc[k + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = axis;
+                    best_bin = k;
+                }
+            }
+        }
+        if (best_axis < 0) return split_median(b, e);
+        const float ext = hi[best_axis] - lo[best_axis];
+        const float sc = NB / ext;
+        const float* cp = cen.data();
+        const float l0 = lo[best_axis];
+        const int ax = best_axis, bin = best_bin;
+        auto mid = std::partition(idx.begin() + b, idx.begin() + e, [cp, ax, l0, sc, bin](int64_t x) {
+            int k = (int)((cp[3 * x + ax] - l0) * sc);
+            k = std::min(std::max(k, 0), NB - 1);
+            return k <= bin;
+        });
+        int64_t m = mid - idx.begin();
+        if (m == b || m == e) return split_median(b, e);
+        return m;
+    }
+
     // Object median on the axis of largest centroid extent.
-    int64_t split(int64_t b, int64_t e) {
+    int64_t split_median(int64_t b, int64_t e) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int64_t i = b; i < e; ++i) {
             const float* c = &cen[3 * idx[i]];
@@ -129,7 +200,8 @@ inline float pad_up(float v, float abs_pad) {
 
 }  // namespace
 
-std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out) {
+std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out,
+                      int split_mode) {
     if (n_faces <= 0) return "mesh has no faces";
     if (n_faces > MAX_LEAF_FIRST) return "too many faces for the 28-bit leaf encoding";
     if (leaf_size < 1 || leaf_size > MAX_LEAF_SIZE) return "leaf_size must be in 1..8";
@@ -137,6 +209,7 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
     Builder B;
     B.tri9 = tri9;
     B.leaf_size = leaf_size;
+    B.split_mode = split_mode;
     B.tbox.resize(n_faces);
     B.cen.resize(3 * n_faces);
     B.idx.resize(n_faces);

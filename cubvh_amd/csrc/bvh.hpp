@@ -61,7 +61,9 @@ struct HostBvh {
 
 // Builds the tree over triangles given as 9 floats each (a, b, c) with original ids.
 // Returns an empty string on success, else an error message.
-std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out);
+// split_mode: 0 object median on the widest centroid axis, 1 binned SAH.
+std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out,
+                      int split_mode = 0);
 
 // Fills bvh.tbnd from bvh.tris (bounding disk of every face, rounded outward).
 void compute_face_bounds(HostBvh& bvh);

@@ -32,6 +32,7 @@ int fail_hip(hipError_t e, const char* what) {
 }
 
 constexpr int kMaxDevices = 64;
+constexpr int kDefaultLeaf = 8;  // faces per leaf unless the caller asks otherwise
 // Point-query batches at least this large are sorted into Morton order and walked as
 // wave packets; smaller ones are walked one lane per query in input order.
 constexpr int64_t kMortonMin = 1 << 11;
@@ -319,7 +320,10 @@ int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* f
     if (!vertices || !faces) return fail(CUBVH_ERR_ARG, "vertices and faces must not be null");
     if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "mesh must have vertices and faces");
     if (n_faces > (int64_t)0x7fffffff) return fail(CUBVH_ERR_ARG, "too many faces");
-    if (leaf_size == 0) leaf_size = 4;
+    const int split_mode = (leaf_size & 0x100) ? 0 : 1;  // bit 8: object median, else SAH
+    if (leaf_size & ~0x1ff) return fail(CUBVH_ERR_ARG, "unknown bits in leaf_size");
+    leaf_size &= 0xff;
+    if (leaf_size == 0) leaf_size = kDefaultLeaf;
     std::vector<float> tri9(9 * (size_t)n_faces);
     std::vector<int64_t> ids((size_t)n_faces);
     for (int64_t f = 0; f < n_faces; ++f) {
@@ -333,7 +337,9 @@ int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* f
         ids[f] = f;
     }
     cubvh_bvh* b = new cubvh_bvh;
-    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host);
+    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host, split_mode);
+    if (err.empty() && split_mode == 1 && stack_capacity_for(b->host.stack_need) < 0)  // SAH tree too deep
+        err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host, 0);
     if (!err.empty()) {
         delete b;
         return fail(CUBVH_ERR_ARG, err);
@@ -424,7 +430,9 @@ int cubvh_bvh_import_reference(const int32_t* nodes9, int64_t n_nodes, const int
             return fail(CUBVH_ERR_STATE, "reference child range out of bounds");
         }
     }
-    if (leaf_size == 0) leaf_size = 4;
+    const int split_mode = (leaf_size & 0x100) ? 0 : 1;
+    leaf_size &= 0xff;
+    if (leaf_size == 0) leaf_size = kDefaultLeaf;
     std::vector<float> tri9(9 * (size_t)n_faces);
     std::vector<int64_t> ids((size_t)n_faces);
     std::vector<uint8_t> seen((size_t)n_faces, 0);
@@ -439,7 +447,9 @@ int cubvh_bvh_import_reference(const int32_t* nodes9, int64_t n_nodes, const int
         ids[s] = id;
     }
     cubvh_bvh* b = new cubvh_bvh;
-    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host);
+    std::string err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host, split_mode);
+    if (err.empty() && split_mode == 1 && stack_capacity_for(b->host.stack_need) < 0)
+        err = build_bvh(tri9.data(), ids.data(), n_faces, leaf_size, b->host, 0);
     if (!err.empty()) {
         delete b;
         return fail(CUBVH_ERR_STATE, err);

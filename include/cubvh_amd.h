@@ -48,7 +48,9 @@ int cubvh_abi_version(void);
  *          TriangleBvh4::build                 src/bvh.cu:515-620.
  * vertices: fp32 [n_vertices, 3] row-major; faces: int32 [n_faces, 3] row-major.
  * Face indices are bounds-checked (the reference does not check them).
- * leaf_size: maximum triangles per leaf, 1..8 (0 selects the default, 4).
+ * leaf_size: bits 0-7 = maximum triangles per leaf, 1..8 (0 selects the default, 8);
+ * bit 8 set = split at the object median instead of the binned surface-area
+ * heuristic (the default).
  * ------------------------------------------------------------------------- */
 int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
                      int32_t leaf_size, cubvh_bvh** out);

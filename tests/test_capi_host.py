@@ -216,3 +216,25 @@ def test_forest_host_concatenation():
     assert vcat.shape[0] == sum(b._vertices.shape[0] for b in bs)
     assert fcat.shape[0] == info["n_faces"] and int(fcat.max()) == vcat.shape[0] - 1
     assert f.face_base().tolist() == [0, 320, 1600]
+
+
+def test_build_options_and_depth_fallback():
+    """Leaf size and split rule flags; a mesh whose SAH tree would be too deep for
+    the traversal stack (geometric cluster sizes) still builds (median fallback)."""
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    v, f = S.icosphere(4)
+    for flags, leaf in ((0, 8), (4, 4), (0x104, 4), (0x101, 1)):
+        info = cubvh_amd.cuBVH(v, f, leaf_size=flags).info()
+        assert info["leaf_size"] <= leaf and info["stack_need"] <= 64
+    # triangles at geometrically shrinking scales: very unbalanced SAH splits
+    tris = []
+    for k in range(3000):
+        s = 0.7 ** (k % 60)
+        c = np.array([s, 0.0, 0.0])
+        tris.append([c, c + [s * 0.1, 0, 0], c + [0, s * 0.1, 0]])
+    V = np.asarray(tris, np.float32).reshape(-1, 3)
+    F = np.arange(len(V), dtype=np.int32).reshape(-1, 3)
+    info = cubvh_amd.cuBVH(V, F).info()
+    assert info["stack_need"] <= 64
