@@ -394,42 +394,91 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
     return "";
 }
 
+namespace {
+
+// Bounding disk of a point set: centre and direction given in double, stored as
+// fp32 (centre, radius, unit normal, thickness).  Radius and thickness are measured
+// from the rounded centre along the rounded normal in double and rounded outward, so
+// every point lies inside the stored cylinder.  A zero direction gives a sphere.
+void store_disk(const double (*v)[3], int nv, const double centre[3], const double dir[3], float* o) {
+    float cf[3], nf[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 3; ++k) cf[k] = (float)centre[k];
+    const double nl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (nl > 0.0 && std::isfinite(nl))
+        for (int k = 0; k < 3; ++k) nf[k] = (float)(dir[k] / nl);
+    double r = 0.0, th = 0.0;
+    for (int i = 0; i < nv; ++i) {
+        const double d[3] = {v[i][0] - cf[0], v[i][1] - cf[1], v[i][2] - cf[2]};
+        const double h = d[0] * nf[0] + d[1] * nf[1] + d[2] * nf[2];
+        const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        r = std::max(r, std::sqrt(std::max(dd - h * h, 0.0)));
+        th = std::max(th, std::fabs(h));
+    }
+    r = r * (1.0 + 1e-6) + 1e-30;
+    th = th * (1.0 + 1e-6) + 1e-30;
+    float rf = (float)r, tf = (float)th;
+    if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
+    if ((double)tf < th) tf = std::nextafter(tf, INFINITY);
+    o[0] = cf[0]; o[1] = cf[1]; o[2] = cf[2]; o[3] = rf;
+    o[4] = nf[0]; o[5] = nf[1]; o[6] = nf[2]; o[7] = tf;
+}
+
+void face_vertices(const HostBvh& bvh, int64_t s, double (*v)[3]) {
+    const float* t = &bvh.tris[12 * s];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) v[i][k] = t[4 * i + k];
+}
+
+void face_normal(const double (*v)[3], double* n) {
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = v[1][k] - v[0][k];
+        e2[k] = v[2][k] - v[0][k];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+}  // namespace
+
 void compute_face_bounds(HostBvh& bvh) {
     const int64_t m = bvh.n_faces;
     bvh.tbnd.assign(8 * (size_t)m, 0.0f);
+    bvh.lbnd.assign(8 * (size_t)m, 0.0f);
+    // faces: centroid and plane normal
     for (int64_t s = 0; s < m; ++s) {
-        const float* t = &bvh.tris[12 * s];
-        double v[3][3] = {{t[0], t[1], t[2]}, {t[4], t[5], t[6]}, {t[8], t[9], t[10]}};
-        float cf[3];
-        for (int k = 0; k < 3; ++k) cf[k] = (float)((v[0][k] + v[1][k] + v[2][k]) / 3.0);
-        double e1[3], e2[3];
-        for (int k = 0; k < 3; ++k) {
-            e1[k] = v[1][k] - v[0][k];
-            e2[k] = v[2][k] - v[0][k];
+        double v[3][3], n[3], c[3];
+        face_vertices(bvh, s, v);
+        face_normal(v, n);
+        for (int k = 0; k < 3; ++k) c[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0;
+        store_disk(v, 3, c, n, &bvh.tbnd[8 * s]);
+    }
+    // leaves: centre of the vertex box and the area-weighted mean normal
+    for (const WideNode& w : bvh.nodes) {
+        for (int k = 0; k < 4; ++k) {
+            const int32_t ref = w.child[k];
+            if (ref == CHILD_EMPTY || ref >= 0) continue;
+            const int64_t first = (int64_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
+            const int count = (ref & 7) + 1;
+            if (first + count > m) continue;  // rejected by validate_bvh
+            double v[3 * MAX_LEAF_SIZE][3], n[3] = {0.0, 0.0, 0.0};
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int j = 0; j < count; ++j) {
+                double fn[3];
+                face_vertices(bvh, first + j, &v[3 * j]);
+                face_normal(&v[3 * j], fn);
+                for (int a = 0; a < 3; ++a) {
+                    n[a] += fn[a];
+                    for (int i = 0; i < 3; ++i) {
+                        lo[a] = std::min(lo[a], v[3 * j + i][a]);
+                        hi[a] = std::max(hi[a], v[3 * j + i][a]);
+                    }
+                }
+            }
+            const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+            store_disk(v, 3 * count, c, n, &bvh.lbnd[8 * first]);
         }
-        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-        double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        float nf[3] = {0.0f, 0.0f, 0.0f};
-        if (nl > 0.0 && std::isfinite(nl))
-            for (int k = 0; k < 3; ++k) nf[k] = (float)(n[k] / nl);
-        // radius (in-plane distance) and thickness (off-plane distance) with respect to
-        // the rounded centre and normal, both from the exact vertex coordinates
-        double r = 0.0, th = 0.0;
-        for (int i = 0; i < 3; ++i) {
-            double d[3] = {v[i][0] - cf[0], v[i][1] - cf[1], v[i][2] - cf[2]};
-            double h = d[0] * nf[0] + d[1] * nf[1] + d[2] * nf[2];
-            double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            r = std::max(r, std::sqrt(std::max(dd - h * h, 0.0)));
-            th = std::max(th, std::fabs(h));
-        }
-        r = r * (1.0 + 1e-6) + 1e-30;
-        th = th * (1.0 + 1e-6) + 1e-30;
-        float rf = (float)r, tf = (float)th;
-        if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
-        if ((double)tf < th) tf = std::nextafter(tf, INFINITY);
-        float* o = &bvh.tbnd[8 * s];
-        o[0] = cf[0]; o[1] = cf[1]; o[2] = cf[2]; o[3] = rf;
-        o[4] = nf[0]; o[5] = nf[1]; o[6] = nf[2]; o[7] = tf;
     }
 }
 

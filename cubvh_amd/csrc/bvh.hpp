@@ -15,6 +15,9 @@
 //           Every point of the face lies within `radius` of the centre in the plane
 //           through the centre with that normal, and within `thickness` of that plane.
 //           It gives a cheap lower bound on the squared distance to the face.
+//   lbnd  : float4[2 * n_faces], the same kind of disk around all faces of a leaf,
+//           stored at the leaf's first slot: lbnd[2f+0..1] for the leaf starting at
+//           triangle f (other slots unused).  One test can skip a whole leaf.
 //
 // Child references (WideNode::child[k]):
 //   ref >= 0                  internal node index
@@ -52,6 +55,7 @@ struct HostBvh {
     std::vector<WideNode> nodes;
     std::vector<float> tris;  // 12 floats per triangle, leaf order, see header comment
     std::vector<float> tbnd;  // 8 floats per triangle, leaf order, see header comment
+    std::vector<float> lbnd;  // 8 floats per triangle slot, set at leaf starts
     int64_t n_faces = 0;
     int32_t depth = 0;        // root = 1
     int32_t stack_need = 0;   // entries a nearest-first traversal may hold
@@ -65,7 +69,8 @@ struct HostBvh {
 std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out,
                       int split_mode = 0);
 
-// Fills bvh.tbnd from bvh.tris (bounding disk of every face, rounded outward).
+// Fills bvh.tbnd and bvh.lbnd from bvh.tris and the leaf references of bvh.nodes
+// (bounding disks of every face and of every leaf, rounded outward).
 void compute_face_bounds(HostBvh& bvh);
 
 // Recomputes depth / stack_need / face0_slot of an imported tree and validates every

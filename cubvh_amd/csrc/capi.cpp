@@ -77,6 +77,7 @@ struct cubvh_bvh {
         WideNode* nodes = nullptr;
         float4* tris = nullptr;
         float4* tbnd = nullptr;
+        float4* lbnd = nullptr;
     };
     DevCopy dev[kMaxDevices];
 
@@ -88,6 +89,7 @@ struct cubvh_bvh {
             (void)hipFree(dev[d].nodes);
             (void)hipFree(dev[d].tris);
             (void)hipFree(dev[d].tbnd);
+            (void)hipFree(dev[d].lbnd);
         }
     }
 
@@ -105,12 +107,15 @@ struct cubvh_bvh {
         WideNode* n = nullptr;
         float4* t = nullptr;
         float4* b = nullptr;
+        float4* l = nullptr;
         hipError_t e = hipMalloc(&n, nb);
         if (e == hipSuccess) e = hipMalloc(&t, tb);
         if (e == hipSuccess) e = hipMalloc(&b, bb);
+        if (e == hipSuccess) e = hipMalloc(&l, bb);
         if (e == hipSuccess) e = hipMemcpyAsync(n, host.nodes.data(), nb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(t, host.tris.data(), tb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(b, host.tbnd.data(), bb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(l, host.lbnd.data(), bb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             (void)hipFree(n);
@@ -121,6 +126,7 @@ struct cubvh_bvh {
         c.nodes = n;
         c.tris = t;
         c.tbnd = b;
+        c.lbnd = l;
         return CUBVH_OK;
     }
 
@@ -130,6 +136,7 @@ struct cubvh_bvh {
         q.nodes = dev[device].nodes;
         q.tris = dev[device].tris;
         q.tbnd = dev[device].tbnd;
+        q.lbnd = dev[device].lbnd;
         q.face0_slot = host.face0_slot;
         q.stack_need = host.stack_need;
         q.stream = stream;
@@ -141,6 +148,7 @@ struct cubvh_forest {
     std::vector<WideNode> nodes;
     std::vector<float> tris;
     std::vector<float> tbnd;
+    std::vector<float> lbnd;
     std::vector<int32_t> mesh_info;  // 4 per mesh: root, face-0 slot, n_faces, 0
     int32_t n_meshes = 0;
     int64_t n_faces = 0;
@@ -150,6 +158,7 @@ struct cubvh_forest {
         WideNode* nodes = nullptr;
         float4* tris = nullptr;
         float4* tbnd = nullptr;
+        float4* lbnd = nullptr;
         int4* mesh_info = nullptr;
         void* ws = nullptr;
         int64_t ws_n = -1;
@@ -167,6 +176,7 @@ struct cubvh_forest {
             (void)hipFree(c.nodes);
             (void)hipFree(c.tris);
             (void)hipFree(c.tbnd);
+            (void)hipFree(c.lbnd);
             (void)hipFree(c.mesh_info);
             (void)hipFree(c.ws);
         }
@@ -182,20 +192,24 @@ struct cubvh_forest {
         e = hipMalloc(&c.nodes, nb);
         if (e == hipSuccess) e = hipMalloc(&c.tris, tb);
         if (e == hipSuccess) e = hipMalloc(&c.tbnd, bb);
+        if (e == hipSuccess) e = hipMalloc(&c.lbnd, bb);
         if (e == hipSuccess) e = hipMalloc(&c.mesh_info, mb);
         if (e == hipSuccess) e = hipMemcpyAsync(c.nodes, nodes.data(), nb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c.tris, tris.data(), tb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c.tbnd, tbnd.data(), bb, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.lbnd, lbnd.data(), bb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c.mesh_info, mesh_info.data(), mb, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             (void)hipFree(c.nodes);
             (void)hipFree(c.tris);
             (void)hipFree(c.tbnd);
+            (void)hipFree(c.lbnd);
             (void)hipFree(c.mesh_info);
             c.nodes = nullptr;
             c.tris = nullptr;
             c.tbnd = nullptr;
+            c.lbnd = nullptr;
             c.mesh_info = nullptr;
             return fail_hip(e, "forest upload");
         }
@@ -585,6 +599,7 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
         }
         f->tris.insert(f->tris.end(), h.tris.begin(), h.tris.end());
         f->tbnd.insert(f->tbnd.end(), h.tbnd.begin(), h.tbnd.end());
+        f->lbnd.insert(f->lbnd.end(), h.lbnd.begin(), h.lbnd.end());
         f->mesh_info[4 * m + 0] = (int32_t)node_base;
         f->mesh_info[4 * m + 1] = (int32_t)(tri_base + h.face0_slot);
         f->mesh_info[4 * m + 2] = (int32_t)h.n_faces;
@@ -675,6 +690,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.nodes = c.nodes;
     a.tris = c.tris;
     a.tbnd = c.tbnd;
+    a.lbnd = c.lbnd;
     a.mesh_info = c.mesh_info;
     a.persistent = persistent;
     a.waves_per_cu = wpc;

@@ -40,6 +40,7 @@ struct ForestArgs {
     const WideNode* nodes = nullptr;
     const float4* tris = nullptr;
     const float4* tbnd = nullptr;
+    const float4* lbnd = nullptr;
     const int4* mesh_info = nullptr;  // per mesh: root node, face-0 slot, n_faces, 0
     int32_t persistent = 0;
     int32_t waves_per_cu = 8;

@@ -36,15 +36,17 @@ struct Tree {
     const WideNode* __restrict__ nodes;
     const float4* __restrict__ tris;
     const float4* __restrict__ tbnd;  // per-face bounding disks
+    const float4* __restrict__ lbnd;  // per-leaf bounding disks, at each leaf's first slot
     int64_t face0_slot;
 };
 
-// True when face `slot` cannot come within squared distance `bound` of p, judged from
-// its bounding disk (centre c, radius r, unit normal n, thickness t): every point x
-// of the face has |p - x|^2 >= max(|h| - t, 0)^2 + max(rho - r, 0)^2 with
-// h = n.(p - c) and rho the in-plane distance from c.  The margin 2^-15 (|p-c|^2 + r^2)
-// exceeds the fp32 rounding of this bound and of the exact face test, so a rejected
-// face could not have produced a squared distance <= bound (nor tied the best).
+// True when no face inside a bounding disk (centre c, radius r, unit normal n,
+// thickness t; one face's or one leaf's) can come within squared distance `bound` of
+// p: every point x inside the disk's cylinder has
+// |p - x|^2 >= max(|h| - t, 0)^2 + max(rho - r, 0)^2 with h = n.(p - c) and rho the
+// in-plane distance from c.  The margin 2^-15 (|p-c|^2 + r^2) exceeds the fp32
+// rounding of this bound and of the exact face test, so a rejected face could not
+// have produced a squared distance <= bound (nor tied the best).
 __device__ __forceinline__ bool disk_rejects(f3 p, float4 cb, float4 nb, float bound) {
     f3 d = mk3(p.x - cb.x, p.y - cb.y, p.z - cb.z);
     float dd = dot3(d, d);
@@ -170,6 +172,7 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
         } else {
             int32_t first, count;
             leaf_range(ref, first, count);
+            if (disk_rejects(p, T.lbnd[2 * first], T.lbnd[2 * first + 1], bound)) count = 0;  // whole leaf out of reach
             for (int j = 0; j < count; ++j) {
                 if (disk_rejects(p, T.tbnd[2 * (first + j)], T.tbnd[2 * (first + j) + 1], bound)) continue;
                 f3 a, b, c;
@@ -497,9 +500,12 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
         } else {
             int32_t first, count;
             leaf_range(ref, first, count);
-            for (int j = 0; j < count; ++j) {
+            // lanes whose point may still reach a face of this leaf
+            cfloat_ptr lp = (cfloat_ptr)(T.lbnd + 2 * first);
+            const uint64_t lm = ballot(!disk_rejects(p, ld4(lp), ld4(lp + 4), bound)) & mask;
+            for (int j = 0; lm != 0 && j < count; ++j) {
                 cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
-                const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & mask;
+                const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & lm;
                 if (cm == 0) continue;  // no lane can reach this face
                 f3 a, b, c;
                 int32_t id;
@@ -683,7 +689,8 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
                                                               const WideNode* __restrict__ nodes,
                                                               const float4* __restrict__ tris,
-                                                              const float4* __restrict__ tbnd, uint32_t* counter) {
+                                                              const float4* __restrict__ tbnd,
+                                                              const float4* __restrict__ lbnd, uint32_t* counter) {
     const int lane = (int)(threadIdx.x & 63);
     const int64_t n_tiles = pbase[n_meshes] / 64;
     int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -707,7 +714,7 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
         const bool valid = qi != 0xffffffffu;
         f3 p = mk3(0.0f, 0.0f, 0.0f);
         if (valid) p = load_pt(positions, qi);
-        Tree T{nodes, tris, tbnd, (int64_t)mi.y};
+        Tree T{nodes, tris, tbnd, lbnd, (int64_t)mi.y};
         float d2;
         int32_t slot, bid;
         closest_face_packet(T, uniform(mi.x), p, valid, d2, slot, bid);
@@ -759,7 +766,7 @@ constexpr int kBlock = 128;
 constexpr int kLdsEntries = 12;
 
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
-    dev::Tree T{q.nodes, q.tris, q.tbnd, q.face0_slot};
+    dev::Tree T{q.nodes, q.tris, q.tbnd, q.lbnd, q.face0_slot};
     hipStream_t s = (hipStream_t)q.stream;
     if ((mode == 0 || mode == 1) && q.packet) {
         const unsigned grid = (unsigned)((q.n + 255) / 256);
@@ -798,13 +805,13 @@ hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws
         const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
                            ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           a.tbnd, ws.counter);
+                           a.tbnd, a.lbnd, ws.counter);
     } else {
         const int64_t max_tiles = ws.npad_max / 64;
         const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
                            ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           a.tbnd, ws.counter);
+                           a.tbnd, a.lbnd, ws.counter);
     }
     return hipGetLastError();
 }
@@ -820,7 +827,7 @@ hipError_t launch_distance(const QueryArgs& q, int mode) {
 hipError_t launch_ray(const QueryArgs& q) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
-    dev::Tree T{q.nodes, q.tris, q.tbnd, q.face0_slot};
+    dev::Tree T{q.nodes, q.tris, q.tbnd, q.lbnd, q.face0_slot};
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((dev::ray_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, (hipStream_t)q.stream, q.n,
                        q.in0, q.in1, q.out_v, q.out_id, q.out_f, T);

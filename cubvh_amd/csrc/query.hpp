@@ -19,6 +19,7 @@ struct QueryArgs {
     const WideNode* nodes = nullptr;
     const float4* tris = nullptr;
     const float4* tbnd = nullptr;  // per-face bounding disks (bvh.hpp)
+    const float4* lbnd = nullptr;  // per-leaf bounding disks at leaf-start slots (bvh.hpp)
     int64_t face0_slot = 0;
     int32_t stack_need = 0;
     bool packet = true;  // walk as wave packets (queries in Morton order) or one lane per query
