@@ -74,8 +74,9 @@ def _time_kernel(fn, steps, warmup):
     return wall, float(np.mean(ev)) / 1e3
 
 
-def cpu_baseline(v, f, pts, sample=1 << 17, threads=None):
-    """Reference BVH walk restated on the CPU (oracle), on a seeded sample of the grid."""
+def cpu_baseline(v, f, pts, sample=1 << 21, sample_1t=1 << 17, threads=None):
+    """Reference BVH walk restated on the CPU (oracle), on seeded samples of the grid:
+    all host threads (up to 16) on `sample` points, one thread on `sample_1t` points."""
     from oracle import oracle as O
 
     threads = threads or min(os.cpu_count() or 1, 16)
@@ -88,14 +89,41 @@ def cpu_baseline(v, f, pts, sample=1 << 17, threads=None):
     t0 = time.perf_counter()
     R.udf(sub, return_uvw=False, nthreads=threads)
     dt = time.perf_counter() - t0
+    sub1 = np.ascontiguousarray(sub[:sample_1t])
+    t0 = time.perf_counter()
+    R.udf(sub1, return_uvw=False, nthreads=1)
+    dt1 = time.perf_counter() - t0
     return {
         "value": sample / dt / 1e6,
         "unit": "Mqueries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample} of the 256^3 grid points (seeded), reference BVH4 build ({build_s:.2f} s, 1 thread) "
-                  f"and closest_triangle walk restated in C++ (oracle/cubvh_oracle.cpp), {threads} OpenMP threads",
+        "value_1thread": sample_1t / dt1 / 1e6,
+        "build_s_1thread": build_s,
+        "sample": f"{sample} of the 256^3 grid points (seeded) on {threads} OpenMP threads ({dt:.1f} s), "
+                  f"the first {sample_1t} of them on 1 thread ({dt1:.1f} s); reference BVH4 build "
+                  f"({build_s:.2f} s, 1 thread) and closest_triangle walk restated in C++ (oracle/cubvh_oracle.cpp)",
     }
+
+
+def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
+    """Device-to-device copy rate (bytes read + written per second): the achievable HBM bandwidth."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / sec / 1e9
 
 
 def fwd_bwd_leg(dev, steps, warmup):
@@ -280,9 +308,11 @@ def main():
     if ab and "udf_level8_grid256" in ab["workloads"] and args.res == 256:
         bpq = ab["workloads"]["udf_level8_grid256"]["bytes_per_query"]
         achieved = bpq * n / ev / 1e9
+        copy_gbs = copy_bandwidth(dev)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic_per_launch(),
-                "bytes_per_query": bpq, "kernel_ms": ev * 1e3}
+                "bytes_per_query": bpq, "kernel_ms": ev * 1e3,
+                "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
     result = {
         "metric": "M UDF queries/s @1.3M tris 256^3 (plain vs fwd+bwd vs batched), % HBM roofline",
         "value": value,

@@ -440,12 +440,37 @@ void face_normal(const double (*v)[3], double* n) {
     n[2] = e1[0] * e2[1] - e1[1] * e2[0];
 }
 
+// fp32 vector operations of the distance formula, with its fused operations and no
+// others (this file is compiled with -ffp-contract=off): bit-identical to the device.
+struct V3 {
+    float x, y, z;
+};
+V3 vsub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+float vdot(V3 a, V3 b) { return std::fmaf(a.z, b.z, std::fmaf(a.y, b.y, a.x * b.x)); }
+V3 vcross(V3 a, V3 b) {
+    return V3{std::fmaf(a.y, b.z, -(a.z * b.y)), std::fmaf(a.z, b.x, -(a.x * b.z)), std::fmaf(a.x, b.y, -(a.y * b.x))};
+}
+
+void store_face_constants(const float* t, float* o) {
+    const V3 a{t[0], t[1], t[2]}, b{t[4], t[5], t[6]}, c{t[8], t[9], t[10]};
+    const V3 v21 = vsub(b, a), v32 = vsub(c, b), v13 = vsub(a, c);
+    const V3 n = vcross(v21, v13);
+    const V3 g1 = vcross(v21, n), g2 = vcross(v32, n), g3 = vcross(v13, n);
+    const float rows[9][4] = {
+        {a.x, a.y, a.z, t[3]},          {b.x, b.y, b.z, vdot(v21, v21)},    {c.x, c.y, c.z, vdot(v32, v32)},
+        {v21.x, v21.y, v21.z, vdot(v13, v13)}, {v32.x, v32.y, v32.z, vdot(n, n)}, {v13.x, v13.y, v13.z, n.x},
+        {g1.x, g1.y, g1.z, n.y},        {g2.x, g2.y, g2.z, n.z},            {g3.x, g3.y, g3.z, 0.0f}};
+    std::memcpy(o, rows, sizeof(rows));
+}
+
 }  // namespace
 
 void compute_face_bounds(HostBvh& bvh) {
     const int64_t m = bvh.n_faces;
     bvh.tbnd.assign(8 * (size_t)m, 0.0f);
     bvh.lbnd.assign(8 * (size_t)m, 0.0f);
+    bvh.tpre.assign(36 * (size_t)m, 0.0f);
+    for (int64_t s = 0; s < m; ++s) store_face_constants(&bvh.tris[12 * s], &bvh.tpre[36 * s]);
     // faces: centroid and plane normal
     for (int64_t s = 0; s < m; ++s) {
         double v[3][3], n[3], c[3];
@@ -480,6 +505,60 @@ void compute_face_bounds(HostBvh& bvh) {
             store_disk(v, 3 * count, c, n, &bvh.lbnd[8 * first]);
         }
     }
+    // internal subtrees: the same rule over every vertex below the child; the nodes are
+    // shared out to threads (each vertex is visited once per ancestor level)
+    const int64_t nn = (int64_t)bvh.nodes.size();
+    bvh.cbnd.assign(32 * (size_t)nn, 0.0f);
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        std::vector<int32_t> st;
+        std::vector<double> pts;  // 3 per vertex
+        for (;;) {
+            const int64_t ni = next.fetch_add(1);
+            if (ni >= nn) return;
+            for (int k = 0; k < 4; ++k) {
+                const int32_t top = bvh.nodes[ni].child[k];
+                if (top < 0) continue;  // leaf or empty slot
+                pts.clear();
+                double n[3] = {0.0, 0.0, 0.0};
+                double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                st.assign(1, top);
+                while (!st.empty()) {
+                    const int32_t ref = st.back();
+                    st.pop_back();
+                    if (ref >= 0) {
+                        for (int c = 0; c < 4; ++c)
+                            if (bvh.nodes[ref].child[c] != CHILD_EMPTY) st.push_back(bvh.nodes[ref].child[c]);
+                        continue;
+                    }
+                    const int64_t first = (int64_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
+                    const int count = (ref & 7) + 1;
+                    for (int j = 0; j < count && first + j < m; ++j) {
+                        double v[3][3], fn[3];
+                        face_vertices(bvh, first + j, v);
+                        face_normal(v, fn);
+                        for (int a = 0; a < 3; ++a) {
+                            n[a] += fn[a];
+                            for (int i = 0; i < 3; ++i) {
+                                lo[a] = std::min(lo[a], v[i][a]);
+                                hi[a] = std::max(hi[a], v[i][a]);
+                            }
+                        }
+                        for (int i = 0; i < 3; ++i) pts.insert(pts.end(), v[i], v[i] + 3);
+                    }
+                }
+                const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+                store_disk(reinterpret_cast<const double(*)[3]>(pts.data()), (int)(pts.size() / 3), c, n,
+                           &bvh.cbnd[32 * ni + 8 * k]);
+            }
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = nn < 64 ? 1u : hw;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
 }
 
 std::string validate_bvh(HostBvh& bvh) {

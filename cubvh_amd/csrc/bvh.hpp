@@ -18,6 +18,14 @@
 //   lbnd  : float4[2 * n_faces], the same kind of disk around all faces of a leaf,
 //           stored at the leaf's first slot: lbnd[2f+0..1] for the leaf starting at
 //           triangle f (other slots unused).  One test can skip a whole leaf.
+//   cbnd  : float4[8 * n_nodes], the same kind of disk around all faces below each
+//           internal child: cbnd[8i + 2k + 0..1] for child slot k of node i (zero
+//           for leaf and empty slots).  Tested with the child's box at the parent.
+//   tpre  : float4[9 * n_faces] in leaf order, the per-face constants of the distance
+//           formula, each computed with the formula's own fp32 operations:
+//           [0] (a, id bits)   [1] (b, |v21|^2)   [2] (c, |v32|^2)   [3] (v21, |v13|^2)
+//           [4] (v32, |n|^2)   [5] (v13, n.x)     [6] (v21 x n, n.y) [7] (v32 x n, n.z)
+//           [8] (v13 x n, 0)   with v21 = b - a, v32 = c - b, v13 = a - c, n = v21 x v13.
 //
 // Child references (WideNode::child[k]):
 //   ref >= 0                  internal node index
@@ -56,6 +64,8 @@ struct HostBvh {
     std::vector<float> tris;  // 12 floats per triangle, leaf order, see header comment
     std::vector<float> tbnd;  // 8 floats per triangle, leaf order, see header comment
     std::vector<float> lbnd;  // 8 floats per triangle slot, set at leaf starts
+    std::vector<float> cbnd;  // 32 floats per node, internal child slots only
+    std::vector<float> tpre;  // 36 floats per triangle, leaf order, see header comment
     int64_t n_faces = 0;
     int32_t depth = 0;        // root = 1
     int32_t stack_need = 0;   // entries a nearest-first traversal may hold
@@ -69,8 +79,10 @@ struct HostBvh {
 std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out,
                       int split_mode = 0);
 
-// Fills bvh.tbnd and bvh.lbnd from bvh.tris and the leaf references of bvh.nodes
-// (bounding disks of every face and of every leaf, rounded outward).
+// Fills the derived arrays from bvh.tris and bvh.nodes: bvh.tbnd, bvh.lbnd and
+// bvh.cbnd (bounding disks of every face, every leaf and every internal subtree,
+// rounded outward) and bvh.tpre (per-face constants of the distance formula).  The node
+// references must already be valid (build_bvh output or validate_bvh passed).
 void compute_face_bounds(HostBvh& bvh);
 
 // Recomputes depth / stack_need / face0_slot of an imported tree and validates every

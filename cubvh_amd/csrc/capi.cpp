@@ -70,26 +70,55 @@ struct DeviceScope {
 
 }  // namespace
 
+// Every array of a tree (or of a forest's concatenated trees) in one device
+// allocation, each part 256-byte aligned.
+struct DeviceTree {
+    void* block = nullptr;
+    TreeArrays view;
+
+    template <class H>
+    hipError_t upload(const H& h, hipStream_t stream) {
+        const void* src[6] = {h.nodes.data(), h.tris.data(), h.tbnd.data(), h.lbnd.data(), h.cbnd.data(), h.tpre.data()};
+        const size_t bytes[6] = {h.nodes.size() * sizeof(WideNode), h.tris.size() * sizeof(float),
+                                 h.tbnd.size() * sizeof(float),      h.lbnd.size() * sizeof(float),
+                                 h.cbnd.size() * sizeof(float),      h.tpre.size() * sizeof(float)};
+        size_t off[6], total = 0;
+        for (int i = 0; i < 6; ++i) {
+            off[i] = total;
+            total += (bytes[i] + 255) & ~(size_t)255;
+        }
+        hipError_t e = hipMalloc(&block, std::max<size_t>(total, 256));
+        for (int i = 0; i < 6; ++i)
+            if (e == hipSuccess && bytes[i])
+                e = hipMemcpyAsync((char*)block + off[i], src[i], bytes[i], hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(block);
+            block = nullptr;
+            return e;
+        }
+        const char* b = (const char*)block;
+        view.nodes = (const WideNode*)(b + off[0]);
+        view.tris = (const float4*)(b + off[1]);
+        view.tbnd = (const float4*)(b + off[2]);
+        view.lbnd = (const float4*)(b + off[3]);
+        view.cbnd = (const float4*)(b + off[4]);
+        view.tpre = (const float4*)(b + off[5]);
+        return hipSuccess;
+    }
+};
+
 struct cubvh_bvh {
     HostBvh host;
     std::mutex mu;
-    struct DevCopy {
-        WideNode* nodes = nullptr;
-        float4* tris = nullptr;
-        float4* tbnd = nullptr;
-        float4* lbnd = nullptr;
-    };
-    DevCopy dev[kMaxDevices];
+    DeviceTree dev[kMaxDevices];
 
     ~cubvh_bvh() {
         for (int d = 0; d < kMaxDevices; ++d) {
-            if (!dev[d].nodes && !dev[d].tris) continue;
+            if (!dev[d].block) continue;
             DeviceScope g(d);
             (void)hipDeviceSynchronize();
-            (void)hipFree(dev[d].nodes);
-            (void)hipFree(dev[d].tris);
-            (void)hipFree(dev[d].tbnd);
-            (void)hipFree(dev[d].lbnd);
+            (void)hipFree(dev[d].block);
         }
     }
 
@@ -97,46 +126,18 @@ struct cubvh_bvh {
     int ensure_device(int device, hipStream_t stream) {
         if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
         std::lock_guard<std::mutex> lock(mu);
-        DevCopy& c = dev[device];
-        if (c.nodes) return CUBVH_OK;
+        if (dev[device].block) return CUBVH_OK;
         DeviceScope g(device);
         if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
-        const size_t nb = host.nodes.size() * sizeof(WideNode);
-        const size_t tb = host.tris.size() * sizeof(float);
-        const size_t bb = host.tbnd.size() * sizeof(float);
-        WideNode* n = nullptr;
-        float4* t = nullptr;
-        float4* b = nullptr;
-        float4* l = nullptr;
-        hipError_t e = hipMalloc(&n, nb);
-        if (e == hipSuccess) e = hipMalloc(&t, tb);
-        if (e == hipSuccess) e = hipMalloc(&b, bb);
-        if (e == hipSuccess) e = hipMalloc(&l, bb);
-        if (e == hipSuccess) e = hipMemcpyAsync(n, host.nodes.data(), nb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(t, host.tris.data(), tb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(b, host.tbnd.data(), bb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(l, host.lbnd.data(), bb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) {
-            (void)hipFree(n);
-            (void)hipFree(t);
-            (void)hipFree(b);
-            return fail_hip(e, "upload");
-        }
-        c.nodes = n;
-        c.tris = t;
-        c.tbnd = b;
-        c.lbnd = l;
+        hipError_t e = dev[device].upload(host, stream);
+        if (e != hipSuccess) return fail_hip(e, "upload");
         return CUBVH_OK;
     }
 
     QueryArgs args(int device, int64_t n, void* stream) const {
         QueryArgs q;
         q.n = n;
-        q.nodes = dev[device].nodes;
-        q.tris = dev[device].tris;
-        q.tbnd = dev[device].tbnd;
-        q.lbnd = dev[device].lbnd;
+        q.tree = dev[device].view;
         q.face0_slot = host.face0_slot;
         q.stack_need = host.stack_need;
         q.stream = stream;
@@ -149,16 +150,15 @@ struct cubvh_forest {
     std::vector<float> tris;
     std::vector<float> tbnd;
     std::vector<float> lbnd;
+    std::vector<float> cbnd;
+    std::vector<float> tpre;
     std::vector<int32_t> mesh_info;  // 4 per mesh: root, face-0 slot, n_faces, 0
     int32_t n_meshes = 0;
     int64_t n_faces = 0;
     int32_t stack_need = 0;
     std::mutex mu;
     struct DevCopy {
-        WideNode* nodes = nullptr;
-        float4* tris = nullptr;
-        float4* tbnd = nullptr;
-        float4* lbnd = nullptr;
+        DeviceTree tree;
         int4* mesh_info = nullptr;
         void* ws = nullptr;
         int64_t ws_n = -1;
@@ -170,13 +170,10 @@ struct cubvh_forest {
     ~cubvh_forest() {
         for (int d = 0; d < kMaxDevices; ++d) {
             DevCopy& c = dev[d];
-            if (!c.nodes && !c.ws) continue;
+            if (!c.tree.block && !c.mesh_info && !c.ws) continue;
             DeviceScope g(d);
             (void)hipDeviceSynchronize();
-            (void)hipFree(c.nodes);
-            (void)hipFree(c.tris);
-            (void)hipFree(c.tbnd);
-            (void)hipFree(c.lbnd);
+            (void)hipFree(c.tree.block);
             (void)hipFree(c.mesh_info);
             (void)hipFree(c.ws);
         }
@@ -184,35 +181,21 @@ struct cubvh_forest {
 
     int upload(int device, hipStream_t stream) {
         DevCopy& c = dev[device];
-        if (c.nodes) return CUBVH_OK;
+        if (c.tree.block) return CUBVH_OK;
         hipError_t e = hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device);
         if (e != hipSuccess) return fail_hip(e, "hipDeviceGetAttribute");
-        const size_t nb = nodes.size() * sizeof(WideNode), tb = tris.size() * sizeof(float),
-                     bb = tbnd.size() * sizeof(float), mb = mesh_info.size() * sizeof(int32_t);
-        e = hipMalloc(&c.nodes, nb);
-        if (e == hipSuccess) e = hipMalloc(&c.tris, tb);
-        if (e == hipSuccess) e = hipMalloc(&c.tbnd, bb);
-        if (e == hipSuccess) e = hipMalloc(&c.lbnd, bb);
-        if (e == hipSuccess) e = hipMalloc(&c.mesh_info, mb);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.nodes, nodes.data(), nb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.tris, tris.data(), tb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.tbnd, tbnd.data(), bb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.lbnd, lbnd.data(), bb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.mesh_info, mesh_info.data(), mb, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) {
-            (void)hipFree(c.nodes);
-            (void)hipFree(c.tris);
-            (void)hipFree(c.tbnd);
-            (void)hipFree(c.lbnd);
-            (void)hipFree(c.mesh_info);
-            c.nodes = nullptr;
-            c.tris = nullptr;
-            c.tbnd = nullptr;
-            c.lbnd = nullptr;
-            c.mesh_info = nullptr;
-            return fail_hip(e, "forest upload");
+        const size_t mb = mesh_info.size() * sizeof(int32_t);
+        if (!c.mesh_info) {
+            e = hipMalloc(&c.mesh_info, mb);
+            if (e == hipSuccess) e = hipMemcpyAsync(c.mesh_info, mesh_info.data(), mb, hipMemcpyHostToDevice, stream);
+            if (e != hipSuccess) {
+                (void)hipFree(c.mesh_info);
+                c.mesh_info = nullptr;
+                return fail_hip(e, "forest upload");
+            }
         }
+        e = c.tree.upload(*this, stream);
+        if (e != hipSuccess) return fail_hip(e, "forest upload");
         return CUBVH_OK;
     }
 
@@ -600,6 +583,8 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
         f->tris.insert(f->tris.end(), h.tris.begin(), h.tris.end());
         f->tbnd.insert(f->tbnd.end(), h.tbnd.begin(), h.tbnd.end());
         f->lbnd.insert(f->lbnd.end(), h.lbnd.begin(), h.lbnd.end());
+        f->cbnd.insert(f->cbnd.end(), h.cbnd.begin(), h.cbnd.end());
+        f->tpre.insert(f->tpre.end(), h.tpre.begin(), h.tpre.end());
         f->mesh_info[4 * m + 0] = (int32_t)node_base;
         f->mesh_info[4 * m + 1] = (int32_t)(tri_base + h.face0_slot);
         f->mesh_info[4 * m + 2] = (int32_t)h.n_faces;
@@ -663,7 +648,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     DeviceScope g(device);
     if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
     cubvh_forest::DevCopy& c = forest->dev[device];
-    if (!c.nodes || !c.ws || c.ws_n < n || c.ws_bits < bits) {
+    if (!c.tree.block || !c.ws || c.ws_n < n || c.ws_bits < bits) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         hipError_t e = hipStreamIsCapturing(hs, &cs);
         if (e != hipSuccess) return fail_hip(e, "hipStreamIsCapturing");
@@ -687,10 +672,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.distances = distances;
     a.face_id = face_id;
     a.uvw = uvw;
-    a.nodes = c.nodes;
-    a.tris = c.tris;
-    a.tbnd = c.tbnd;
-    a.lbnd = c.lbnd;
+    a.tree = c.tree.view;
     a.mesh_info = c.mesh_info;
     a.persistent = persistent;
     a.waves_per_cu = wpc;

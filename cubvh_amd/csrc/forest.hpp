@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "bvh.hpp"
+#include "query.hpp"
 
 namespace cubvh_amd {
 
@@ -37,10 +38,7 @@ struct ForestArgs {
     float* distances = nullptr;
     int64_t* face_id = nullptr;
     float* uvw = nullptr;
-    const WideNode* nodes = nullptr;
-    const float4* tris = nullptr;
-    const float4* tbnd = nullptr;
-    const float4* lbnd = nullptr;
+    TreeArrays tree;
     const int4* mesh_info = nullptr;  // per mesh: root node, face-0 slot, n_faces, 0
     int32_t persistent = 0;
     int32_t waves_per_cu = 8;

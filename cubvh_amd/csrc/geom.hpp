@@ -72,6 +72,58 @@ __device__ __forceinline__ float tri_dist2(f3 a, f3 b, f3 c, f3 p) {
     return q * q / dot3(nor, nor);
 }
 
+// The face-only terms of tri_dist2, computed once per face with the same operations
+// (edges v21 = b - a, v32 = c - b, v13 = a - c; nor = v21 x v13; g_k = v_k x nor; the
+// squared lengths are dot3 of each vector with itself).
+struct FaceConst {
+    f3 a, b, c, v21, v32, v13, nor, g1, g2, g3;
+    float l21, l32, l13, lnn;
+    int32_t id;
+};
+
+// tri_dist2 over precomputed face terms: the same operations on the same values, so
+// the same result bit for bit.
+__device__ __forceinline__ float tri_dist2_pre(const FaceConst& f, f3 p) {
+    f3 p1 = sub3(p, f.a), p2 = sub3(p, f.b), p3 = sub3(p, f.c);
+    float s = sgn(dot3(f.g1, p1)) + sgn(dot3(f.g2, p2)) + sgn(dot3(f.g3, p3));
+    if (s < 2.0f) {
+        float t1 = clamp01(dot3(f.v21, p1) / f.l21);
+        float t2 = clamp01(dot3(f.v32, p2) / f.l32);
+        float t3 = clamp01(dot3(f.v13, p3) / f.l13);
+        f3 e1 = mk3(__builtin_fmaf(f.v21.x, t1, -p1.x), __builtin_fmaf(f.v21.y, t1, -p1.y),
+                    __builtin_fmaf(f.v21.z, t1, -p1.z));
+        f3 e2 = mk3(__builtin_fmaf(f.v32.x, t2, -p2.x), __builtin_fmaf(f.v32.y, t2, -p2.y),
+                    __builtin_fmaf(f.v32.z, t2, -p2.z));
+        f3 e3 = mk3(__builtin_fmaf(f.v13.x, t3, -p3.x), __builtin_fmaf(f.v13.y, t3, -p3.y),
+                    __builtin_fmaf(f.v13.z, t3, -p3.z));
+        return smin(smin(dot3(e1, e1), dot3(e2, e2)), dot3(e3, e3));
+    }
+    float q = dot3(f.nor, p1);
+    return q * q / f.lnn;
+}
+
+// Unpacks a 36-float face record (bvh.hpp: tpre) given as nine float4 rows.
+__device__ __forceinline__ FaceConst face_const(float4 r0, float4 r1, float4 r2, float4 r3, float4 r4, float4 r5,
+                                                float4 r6, float4 r7, float4 r8) {
+    FaceConst f;
+    f.a = xyz(r0);
+    f.id = __float_as_int(r0.w);
+    f.b = xyz(r1);
+    f.l21 = r1.w;
+    f.c = xyz(r2);
+    f.l32 = r2.w;
+    f.v21 = xyz(r3);
+    f.l13 = r3.w;
+    f.v32 = xyz(r4);
+    f.lnn = r4.w;
+    f.v13 = xyz(r5);
+    f.g1 = xyz(r6);
+    f.g2 = xyz(r7);
+    f.g3 = xyz(r8);
+    f.nor = mk3(r5.w, r6.w, r7.w);
+    return f;
+}
+
 // Ray parameter of the hit with triangle abc, 1e6 on a miss.  The reference takes
 // 1/(d.n) through safe_divide, which clamps |d.n| at 1e-6 and on grazing rays scales
 // u, v and t by |d.n|/1e-6 (reporting hits far from the triangle, or at the wrong

@@ -32,13 +32,16 @@ constexpr float NORMAL_EPS = 1e-6f;                  // src/bvh.cu:360
 // visited, which absorbs fp32 rounding differences between box and triangle tests.
 constexpr float BOUND_SLACK = 1.0f + 1.0f / 4096.0f;
 
-struct Tree {
-    const WideNode* __restrict__ nodes;
-    const float4* __restrict__ tris;
-    const float4* __restrict__ tbnd;  // per-face bounding disks
-    const float4* __restrict__ lbnd;  // per-leaf bounding disks, at each leaf's first slot
+struct Tree : TreeArrays {
     int64_t face0_slot;
 };
+
+__host__ __device__ inline Tree make_tree(const TreeArrays& a, int64_t face0_slot) {
+    Tree t;
+    static_cast<TreeArrays&>(t) = a;
+    t.face0_slot = face0_slot;
+    return t;
+}
 
 // True when no face inside a bounding disk (centre c, radius r, unit normal n,
 // thickness t; one face's or one leaf's) can come within squared distance `bound` of
@@ -57,6 +60,11 @@ __device__ __forceinline__ bool disk_rejects(f3 p, float4 cb, float4 nb, float b
     float lb = ah * ah + t * t;
     float margin = 3.0517578125e-05f * (dd + cb.w * cb.w);
     return lb - margin > bound;
+}
+
+__device__ __forceinline__ FaceConst load_face(const float4* __restrict__ tpre, int32_t slot) {
+    const float4* r = tpre + 9 * slot;
+    return face_const(r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]);
 }
 
 __device__ __forceinline__ void load_tri(const float4* __restrict__ tris, int32_t slot, f3& a, f3& b, f3& c,
@@ -161,6 +169,9 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
             float k[4];
             int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
             box_dist2_4(nd, p, k);
+            for (int j = 0; j < 4; ++j)  // internal subtrees whose disk is out of reach
+                if (r[j] >= 0 && k[j] <= bound && disk_rejects(p, T.cbnd[8 * ref + 2 * j], T.cbnd[8 * ref + 2 * j + 1], bound))
+                    k[j] = INFINITY;
             sort4(k, r);
             if (k[0] <= bound) {
                 if (k[3] <= bound) st.push(r[3], k[3]);
@@ -175,10 +186,9 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
             if (disk_rejects(p, T.lbnd[2 * first], T.lbnd[2 * first + 1], bound)) count = 0;  // whole leaf out of reach
             for (int j = 0; j < count; ++j) {
                 if (disk_rejects(p, T.tbnd[2 * (first + j)], T.tbnd[2 * (first + j) + 1], bound)) continue;
-                f3 a, b, c;
-                int32_t id;
-                load_tri(T.tris, first + j, a, b, c, id);
-                float d2 = tri_dist2(a, b, c, p);
+                const FaceConst fc = load_face(T.tpre, first + j);
+                const float d2 = tri_dist2_pre(fc, p);
+                const int32_t id = fc.id;
                 if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
                     best = d2;
                     bslot = first + j;
@@ -406,6 +416,12 @@ __device__ __forceinline__ void load_utri(const float4* tris, int32_t slot, f3& 
     id = __float_as_int(t0.w);
 }
 
+__device__ __forceinline__ FaceConst load_uface(const float4* tpre, int32_t slot) {
+    cfloat_ptr r = (cfloat_ptr)(tpre + 9 * slot);
+    return face_const(ld4(r), ld4(r + 4), ld4(r + 8), ld4(r + 12), ld4(r + 16), ld4(r + 20), ld4(r + 24),
+                      ld4(r + 28), ld4(r + 32));
+}
+
 __device__ __forceinline__ float box_d2(float lx, float hx, float ly, float hy, float lz, float hz, f3 p) {
     float dx = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
     float dy = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
@@ -457,10 +473,70 @@ __device__ __forceinline__ uint32_t rep_key(float k, uint64_t m) {
     return m ? v : 0x7f800000u;
 }
 
+// Exact face tests of a wave, queued in LDS as (face slot, query lane) pairs and run
+// 64 at a time with one pair per lane, so that a test keeps every lane busy however
+// few lanes wanted the face.  QCAP holds a full round plus one face's worth of pairs.
+constexpr int QCAP = 128;
+struct ExactQueue {
+    uint32_t slot[QCAP];
+    uint32_t lane[QCAP];
+    unsigned long long key[64];  // per query lane: (squared distance bits, face id), min wins
+    int32_t kslot[64];           // per query lane: slot of the face holding key
+};
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Lanes below `lane` in m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Runs the newest min(qn, 64) queued pairs and folds their results into each query
+// lane's best.  Squared distances are >= 0 (or NaN / +inf, which never win), so their
+// bit patterns order like the values, and a key (d2 bits, face id) orders by distance
+// and then by the lower face id: the walk's tie rule.  An unset best (bslot < 0) has
+// the key (MAX_DIST_SQ, 0), so a face must come strictly closer than MAX_DIST_SQ.
+__device__ __forceinline__ void run_exact(const Tree& T, ExactQueue& q, int32_t& qn, f3 p, float& best,
+                                          float& bound, int32_t& bslot, int32_t& bid) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t take = qn < 64 ? qn : 64;
+    const int32_t base = qn - take;
+    const unsigned long long mine =
+        ((unsigned long long)__float_as_uint(best) << 32) | (uint32_t)(bslot >= 0 ? bid : 0);
+    q.key[lane] = mine;
+    wave_fence();
+    const bool has = lane < take;
+    uint32_t slot = 0, src = 0;
+    if (has) {
+        slot = q.slot[base + lane];
+        src = q.lane[base + lane];
+    }
+    const f3 ps = mk3(__shfl(p.x, (int)src), __shfl(p.y, (int)src), __shfl(p.z, (int)src));
+    unsigned long long k = 0;
+    if (has) {
+        const FaceConst fc = load_face(T.tpre, (int32_t)slot);
+        k = ((unsigned long long)__float_as_uint(tri_dist2_pre(fc, ps)) << 32) | (uint32_t)fc.id;
+        atomicMin(&q.key[src], k);
+    }
+    wave_fence();
+    if (has && q.key[src] == k) q.kslot[src] = (int32_t)slot;
+    wave_fence();
+    const unsigned long long now = q.key[lane];
+    if (now != mine) {
+        best = __uint_as_float((uint32_t)(now >> 32));
+        bid = (int32_t)(uint32_t)now;
+        bslot = q.kslot[lane];
+        bound = best * BOUND_SLACK;
+    }
+    qn = base;
+    wave_fence();
+}
+
 // Nearest face for every lane of the wave (lanes with active = false only ride
-// along).  Results in the lane's best / bslot / bid, as closest_face.
+// along).  Results in the lane's best / bslot / bid, as closest_face.  q is this
+// wave's own LDS queue.
 __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool active, float& best, int32_t& bslot,
-                                    int32_t& bid) {
+                                    int32_t& bid, ExactQueue& q) {
     const int lane = (int)(threadIdx.x & 63);
     best = MAX_DIST_SQ;
     float bound = MAX_DIST_SQ * BOUND_SLACK;
@@ -469,6 +545,7 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
     uint64_t mask = ballot(active);
     if (mask == 0) return;
     WaveStack st;
+    int32_t qn = 0;  // queued pairs (wave-uniform)
     int32_t ref = uniform(root);
     for (;;) {
         if (ref >= 0) {
@@ -478,6 +555,12 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
             uint64_t m1 = ballot(box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, p) <= bound) & mask;
             uint64_t m2 = ballot(box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, p) <= bound) & mask;
             uint64_t m3 = ballot(box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, p) <= bound) & mask;
+            // internal subtrees: drop the lanes that the subtree's disk puts out of reach
+            cfloat_ptr cp = (cfloat_ptr)(T.cbnd + 8 * ref);
+            if (m0 != 0 && nd.child.x >= 0) m0 = ballot(!disk_rejects(p, ld4(cp), ld4(cp + 4), bound)) & m0;
+            if (m1 != 0 && nd.child.y >= 0) m1 = ballot(!disk_rejects(p, ld4(cp + 8), ld4(cp + 12), bound)) & m1;
+            if (m2 != 0 && nd.child.z >= 0) m2 = ballot(!disk_rejects(p, ld4(cp + 16), ld4(cp + 20), bound)) & m2;
+            if (m3 != 0 && nd.child.w >= 0) m3 = ballot(!disk_rejects(p, ld4(cp + 24), ld4(cp + 28), bound)) & m3;
             if ((m0 | m1 | m2 | m3) != 0) {
                 // order the children by the key of their lowest interested lane
                 uint32_t r0 = rep_key(box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, p), m0);
@@ -507,22 +590,24 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
                 cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
                 const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & lm;
                 if (cm == 0) continue;  // no lane can reach this face
-                f3 a, b, c;
-                int32_t id;
-                load_utri(T.tris, first + j, a, b, c, id);
                 if ((cm >> lane) & 1) {
-                    float d2 = tri_dist2(a, b, c, p);
-                    if (d2 < best || (d2 == best && bslot >= 0 && id < bid)) {
-                        best = d2;
-                        bslot = first + j;
-                        bid = id;
-                        bound = best * BOUND_SLACK;
-                    }
+                    const uint32_t at = (uint32_t)qn + lanes_below(cm);
+                    q.slot[at] = (uint32_t)(first + j);
+                    q.lane[at] = (uint32_t)lane;
+                }
+                qn += __builtin_popcountll(cm);
+                if (qn >= 64) {
+                    wave_fence();
+                    run_exact(T, q, qn, p, best, bound, bslot, bid);
                 }
             }
         }
         if (!st.pop(ref, mask)) break;
         ref = uniform(ref);
+    }
+    if (qn > 0) {
+        wave_fence();
+        run_exact(T, q, qn, p, best, bound, bslot, bid);
     }
 }
 
@@ -653,7 +738,8 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
     }
     float d2;
     int32_t slot, bid;
-    closest_face_packet(T, 0, p, valid, d2, slot, bid);
+    __shared__ ExactQueue queues[4];
+    closest_face_packet(T, 0, p, valid, d2, slot, bid, queues[threadIdx.x >> 6]);
     if (slot < 0) {  // nothing within MAX_DIST: distance 0 and face 0
         slot = (int32_t)T.face0_slot;
         d2 = 0.0f;
@@ -687,10 +773,8 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
                                                               const int4* __restrict__ mesh_info,
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
-                                                              const WideNode* __restrict__ nodes,
-                                                              const float4* __restrict__ tris,
-                                                              const float4* __restrict__ tbnd,
-                                                              const float4* __restrict__ lbnd, uint32_t* counter) {
+                                                              TreeArrays arrays, uint32_t* counter) {
+    __shared__ ExactQueue queues[4];
     const int lane = (int)(threadIdx.x & 63);
     const int64_t n_tiles = pbase[n_meshes] / 64;
     int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -714,10 +798,10 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
         const bool valid = qi != 0xffffffffu;
         f3 p = mk3(0.0f, 0.0f, 0.0f);
         if (valid) p = load_pt(positions, qi);
-        Tree T{nodes, tris, tbnd, lbnd, (int64_t)mi.y};
+        const Tree T = make_tree(arrays, (int64_t)mi.y);
         float d2;
         int32_t slot, bid;
-        closest_face_packet(T, uniform(mi.x), p, valid, d2, slot, bid);
+        closest_face_packet(T, uniform(mi.x), p, valid, d2, slot, bid, queues[threadIdx.x >> 6]);
         if (valid) {
             if (slot < 0) {
                 slot = mi.y;
@@ -725,7 +809,7 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
             }
             f3 a, b, c;
             int32_t id;
-            load_tri(tris, slot, a, b, c, id);
+            load_tri(T.tris, slot, a, b, c, id);
             distances[qi] = sqrtf(d2);
             face_id[qi] = (int64_t)id;
             if (uvw) store_pt(uvw, qi, tri_barycentric(a, b, c, tri_closest_pt(a, b, c, p)));
@@ -766,7 +850,7 @@ constexpr int kBlock = 128;
 constexpr int kLdsEntries = 12;
 
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
-    dev::Tree T{q.nodes, q.tris, q.tbnd, q.lbnd, q.face0_slot};
+    const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     hipStream_t s = (hipStream_t)q.stream;
     if ((mode == 0 || mode == 1) && q.packet) {
         const unsigned grid = (unsigned)((q.n + 255) / 256);
@@ -804,14 +888,12 @@ hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws
     if (a.persistent) {
         const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           a.tbnd, a.lbnd, ws.counter);
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter);
     } else {
         const int64_t max_tiles = ws.npad_max / 64;
         const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.nodes, a.tris,
-                           a.tbnd, a.lbnd, ws.counter);
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter);
     }
     return hipGetLastError();
 }
@@ -827,7 +909,7 @@ hipError_t launch_distance(const QueryArgs& q, int mode) {
 hipError_t launch_ray(const QueryArgs& q) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
-    dev::Tree T{q.nodes, q.tris, q.tbnd, q.lbnd, q.face0_slot};
+    const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((dev::ray_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, (hipStream_t)q.stream, q.n,
                        q.in0, q.in1, q.out_v, q.out_id, q.out_f, T);

@@ -8,6 +8,16 @@
 
 namespace cubvh_amd {
 
+// Device arrays of one tree (or of a forest's concatenated trees); layouts in bvh.hpp.
+struct TreeArrays {
+    const WideNode* nodes = nullptr;
+    const float4* tris = nullptr;  // leaf-ordered triangles
+    const float4* tbnd = nullptr;  // per-face bounding disks
+    const float4* lbnd = nullptr;  // per-leaf bounding disks at leaf-start slots
+    const float4* cbnd = nullptr;  // per-child bounding disks of internal subtrees
+    const float4* tpre = nullptr;  // per-face constants of the distance formula
+};
+
 struct QueryArgs {
     int64_t n = 0;
     const float* in0 = nullptr;  // positions or ray origins [n,3]
@@ -16,10 +26,7 @@ struct QueryArgs {
     float* out_f = nullptr;      // distances or depth [n]
     int64_t* out_id = nullptr;   // face ids [n]
     float* out_v = nullptr;      // uvw or hit positions [n,3]
-    const WideNode* nodes = nullptr;
-    const float4* tris = nullptr;
-    const float4* tbnd = nullptr;  // per-face bounding disks (bvh.hpp)
-    const float4* lbnd = nullptr;  // per-leaf bounding disks at leaf-start slots (bvh.hpp)
+    TreeArrays tree;
     int64_t face0_slot = 0;
     int32_t stack_need = 0;
     bool packet = true;  // walk as wave packets (queries in Morton order) or one lane per query
