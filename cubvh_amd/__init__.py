@@ -5,7 +5,9 @@ Drop-in for the reference package's exports (cubvh/__init__.py:1-11): use
 libcubvh_amd.so; the other reference names raise NotImplementedError.
 Additions: :func:`udf` (differentiable unsigned distance), :class:`cuBVHForest`
 (one launch over many meshes), :class:`ForestGraph` (HIP-graph replay of a forest
-query), :func:`udf_batched`, :func:`tune_forest` (launch autotuner).
+query), :func:`udf_batched`, :func:`tune_forest` (launch autotuner),
+:func:`morton_order` + ``cuBVH.distance_ordered`` (order a point set once, query it
+many times).
 """
 from .api import (
     cuBVH,
@@ -18,6 +20,7 @@ from .api import (
     merge_vertices,
     decimate,
     parallel_decimate,
+    morton_order,
 )
 from .autograd import udf
 from .forest import cuBVHForest, ForestGraph, udf_batched
@@ -39,4 +42,5 @@ __all__ = [
     "ForestGraph",
     "udf_batched",
     "tune_forest",
+    "morton_order",
 ]

@@ -26,6 +26,11 @@ SIGNATURES = {
     "cubvh_bvh_triangles": (c_int, [c_void_p, c_void_p]),
     "cubvh_bvh_upload": (c_int, [c_void_p, c_int, c_void_p]),
     "cubvh_unsigned_distance": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "cubvh_morton_order": (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p]),
+    "cubvh_distance_ordered": (
+        c_int,
+        [c_void_p, c_int, c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "cubvh_signed_distance": (
         c_int,
         [c_void_p, c_int, c_int64, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -168,6 +168,25 @@ class cuBVH:
         (normal averaging at the closest point) or 'raystab' (64 stab rays)."""
         return self._distance(positions, return_uvw, _sdf_mode_to_id[mode])
 
+    def distance_ordered(self, positions, order, return_uvw=False, signed=False):
+        """unsigned_distance (or watertight signed_distance with signed=True) over a
+        query order from morton_order(positions), which a caller querying the same
+        points repeatedly computes once.  positions [n, 3] on the GPU."""
+        positions = _query_tensor(positions).view(-1, 3)
+        n = positions.shape[0]
+        dev = positions.device
+        if order.dtype != torch.int32 or order.shape != (n,) or order.device != dev:
+            raise ValueError("order must be an int32 tensor [n] on the device of positions (from morton_order)")
+        order = order.contiguous()
+        distances = torch.empty(n, dtype=torch.float32, device=dev)
+        face_id = torch.empty(n, dtype=torch.int64, device=dev)
+        uvw = torch.empty(n, 3, dtype=torch.float32, device=dev) if return_uvw else None
+        _lib.check(self._lib.cubvh_distance_ordered(self._h, dev.index, n, 1 if signed else 0, positions.data_ptr(),
+                                                    order.data_ptr(), distances.data_ptr(), face_id.data_ptr(),
+                                                    uvw.data_ptr() if uvw is not None else None,
+                                                    _stream_of(positions)))
+        return distances, face_id, uvw
+
     def _distance(self, positions, return_uvw, mode):
         positions = _query_tensor(positions)
         prefix = positions.shape[:-1]
@@ -191,6 +210,18 @@ class cuBVH:
         if uvw is not None:
             uvw = uvw.view(*prefix, 3)
         return distances, face_id, uvw
+
+
+def morton_order(positions):
+    """Permutation (int32 [n]) that puts query points [n, 3] in Morton order of their
+    bounding box, for cuBVH.distance_ordered."""
+    positions = _query_tensor(positions).view(-1, 3)
+    n = positions.shape[0]
+    order = torch.empty(n, dtype=torch.int32, device=positions.device)
+    lib = _lib.load()
+    _lib.check(lib.cubvh_morton_order(positions.device.index, n, positions.data_ptr(), order.data_ptr(),
+                                      _stream_of(positions)))
+    return order
 
 
 def floodfill(grid):

@@ -482,6 +482,55 @@ int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* 
     return run_distance(bvh, device, n, 0, positions, distances, face_id, uvw, stream);
 }
 
+int cubvh_morton_order(int device, int64_t n, const float* positions, uint32_t* order, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    if (n == 0) return CUBVH_OK;
+    if (!positions || !order) return fail(CUBVH_ERR_ARG, "null query buffer");
+    if (n >= ((int64_t)1 << 31)) return fail(CUBVH_ERR_ARG, "too many queries to order (at most 2^31 - 1)");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    int rc = ensure_pool(device);
+    if (rc != CUBVH_OK) return rc;
+    hipStream_t hs = (hipStream_t)stream;
+    void* ws = nullptr;
+    hipError_t e = hipMallocAsync(&ws, morton_workspace_bytes(n), hs);
+    if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(ordering workspace)");
+    e = morton_order(n, positions, ws, order, hs);
+    hipError_t e2 = hipFreeAsync(ws, hs);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail_hip(e, "query ordering");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_distance_ordered(cubvh_bvh* bvh, int device, int64_t n, int32_t kind, const float* positions,
+                           const uint32_t* order, float* distances, int64_t* face_id, float* uvw, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh) return fail(CUBVH_ERR_ARG, "bvh is null");
+    if (kind != 0 && kind != 1) return fail(CUBVH_ERR_ARG, "kind must be 0 (unsigned) or 1 (watertight)");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    if (n == 0) return CUBVH_OK;
+    if (!positions || !order || !distances || !face_id) return fail(CUBVH_ERR_ARG, "null query buffer");
+    if (n >= ((int64_t)1 << 31)) return fail(CUBVH_ERR_ARG, "too many queries (at most 2^31 - 1 per ordered call)");
+    int rc = bvh->ensure_device(device, (hipStream_t)stream);
+    if (rc != CUBVH_OK) return rc;
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    QueryArgs q = bvh->args(device, n, stream);
+    q.in0 = positions;
+    q.order = order;
+    q.out_f = distances;
+    q.out_id = face_id;
+    q.out_v = uvw;
+    q.packet = true;
+    hipError_t e = launch_distance(q, kind);
+    if (e != hipSuccess) return fail_hip(e, "distance kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
 int cubvh_signed_distance(cubvh_bvh* bvh, int device, int64_t n, uint32_t mode, const float* positions,
                           float* distances, int64_t* face_id, float* uvw, void* stream) {
     // src/bvh.cu:440-465: mode 0 is watertight, every other value raystab

@@ -729,13 +729,11 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
                                                               Tree T) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
-    int64_t q = 0;
+    int64_t q = i;
+    if (i < n && order) q = (int64_t)order[i];
+    const bool valid = i < n && q < n;  // order entries >= n are skipped
     f3 p = mk3(0.0f, 0.0f, 0.0f);
-    if (valid) {
-        q = order ? (int64_t)order[i] : i;
-        p = load_pt(positions, q);
-    }
+    if (valid) p = load_pt(positions, q);
     float d2;
     int32_t slot, bid;
     __shared__ ExactQueue queues[4];

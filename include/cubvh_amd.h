@@ -107,6 +107,20 @@ int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* 
 int cubvh_signed_distance(cubvh_bvh* bvh, int device, int64_t n, uint32_t mode, const float* positions,
                           float* distances, int64_t* face_id, float* uvw, void* stream);
 
+/* The two steps of cubvh_unsigned_distance / watertight cubvh_signed_distance for
+ * callers that query one point set repeatedly (no counterpart in the reference,
+ * whose kernels take queries in input order).  cubvh_morton_order writes a
+ * permutation of 0..n-1 (uint32 [n], n < 2^31) that puts the points in Morton order
+ * of their bounding box; cubvh_distance_ordered walks the queries in the given
+ * order as wave packets.  kind 0 = unsigned, 1 = signed (watertight).  Results are
+ * identical to the unordered calls for any permutation; order entries >= n are
+ * skipped (their queries are not written).  Neither call keeps a reference to the
+ * caller's buffers; cubvh_morton_order allocates stream-ordered scratch,
+ * cubvh_distance_ordered allocates nothing. */
+int cubvh_morton_order(int device, int64_t n, const float* positions, uint32_t* order, void* stream);
+int cubvh_distance_ordered(cubvh_bvh* bvh, int device, int64_t n, int32_t kind, const float* positions,
+                           const uint32_t* order, float* distances, int64_t* face_id, float* uvw, void* stream);
+
 /* Replaces cuBVH::ray_trace (src/api_gpu.cu:70-81, src/bvh.cu:490-513,
  * raytrace_kernel src/bvh.cu:714-740).  rays_o, rays_d, positions fp32 [n,3];
  * face_id int64 [n] (-1 on a miss); depth fp32 [n] (1000 on a miss). */

@@ -238,3 +238,19 @@ def test_build_options_and_depth_fallback():
     F = np.arange(len(V), dtype=np.int32).reshape(-1, 3)
     info = cubvh_amd.cuBVH(V, F).info()
     assert info["stack_need"] <= 64
+
+
+def test_ordered_entry_points_validate_arguments(ico4):
+    """Argument errors are reported before any device work."""
+    import cubvh_amd
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    v, f = ico4
+    b = cubvh_amd.cuBVH(v, f)
+    assert lib.cubvh_distance_ordered(b._h, 0, 10, 2, 1, 1, 1, 1, None, None) == -1
+    assert b"kind" in lib.cubvh_last_error()
+    assert lib.cubvh_distance_ordered(b._h, 0, -1, 0, 1, 1, 1, 1, None, None) == -1
+    assert lib.cubvh_morton_order(0, 1 << 31, 1, 1, None) == -1
+    assert b"at most" in lib.cubvh_last_error()
+    assert lib.cubvh_morton_order(0, 0, None, None, None) == 0

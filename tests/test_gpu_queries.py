@@ -202,3 +202,39 @@ def test_state_dict_roundtrip_on_gpu(gpu, oracle, ico4):
     c = ref.unsigned_distance(p, return_uvw=True)
     for x, y, z in zip(a, b, c):
         assert torch.equal(x, y) and torch.equal(x, z)
+
+
+def test_ordered_distance_equals_plain_calls(gpu, oracle):
+    """morton_order + distance_ordered give the plain calls' results for the Morton
+    permutation, for an arbitrary permutation, and skip order entries >= n."""
+    v, f, _ = S.perturbed_sphere(5, seed=3)
+    p = torch.from_numpy(S.uniform_points(40000, seed=4)).to(gpu)
+    bvh = cubvh_amd.cuBVH(v, f)
+    d0, f0, u0 = bvh.unsigned_distance(p, return_uvw=True)
+    s0, g0, w0 = bvh.signed_distance(p, return_uvw=True, mode="watertight")
+    order = cubvh_amd.morton_order(p)
+    assert torch.equal(torch.sort(order.long()).values, torch.arange(p.shape[0], device=gpu))
+    for perm in (order, torch.randperm(p.shape[0], device=gpu).to(torch.int32)):
+        d, fi, u = bvh.distance_ordered(p, perm, return_uvw=True)
+        assert torch.equal(d, d0) and torch.equal(fi, f0) and torch.equal(u, u0)
+        s, g, w = bvh.distance_ordered(p, perm, return_uvw=True, signed=True)
+        assert torch.equal(s, s0) and torch.equal(g, g0) and torch.equal(w, w0)
+    od, of, _ = oracle.udf_brute(v, f, _np(p)[:2000])
+    np.testing.assert_array_equal(_np(d0)[:2000], od)
+    np.testing.assert_array_equal(_np(f0)[:2000], of)
+    # an out-of-range entry leaves its query unwritten and the others exact
+    bad = order.clone()
+    victim = int(bad[7])
+    bad[7] = p.shape[0] + 5
+    lib = cubvh_amd._lib.load()
+    n = p.shape[0]
+    dist = torch.full((n,), -7.0, device=gpu)
+    face = torch.full((n,), -9, dtype=torch.int64, device=gpu)
+    cubvh_amd._lib.check(lib.cubvh_distance_ordered(bvh._h, gpu.index, n, 0, p.data_ptr(), bad.data_ptr(),
+                                                    dist.data_ptr(), face.data_ptr(), None,
+                                                    torch.cuda.current_stream(gpu).cuda_stream))
+    torch.cuda.synchronize()
+    assert float(dist[victim]) == -7.0 and int(face[victim]) == -9
+    keep = torch.ones(n, dtype=torch.bool, device=gpu)
+    keep[victim] = False
+    assert torch.equal(dist[keep], d0[keep]) and torch.equal(face[keep], f0[keep])
