@@ -291,7 +291,14 @@ int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* 
         q.order = order;
     }
     q.packet = scratch != nullptr;
-    hipError_t e = launch_distance(q, kmode);
+    hipError_t e;
+    if (kmode == 2 && q.packet) {
+        // raystab over Morton-ordered queries: packet distance walk, then the stab rays
+        e = launch_distance(q, 0);
+        if (e == hipSuccess) e = launch_stab_sign(q);
+    } else {
+        e = launch_distance(q, kmode);
+    }
     if (scratch) {
         hipError_t e2 = hipFreeAsync(scratch, hs);
         if (e == hipSuccess) e = e2;

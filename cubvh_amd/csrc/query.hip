@@ -347,7 +347,10 @@ __device__ f3 avg_normal(const Tree& T, f3 c, LaneStack<BLOCK, LDS_N>& st) {
         } else {
             int32_t first, count;
             leaf_range(ref, first, count);
+            // disks that put a leaf or face beyond NORMAL_EPS rule out d2 < NORMAL_EPS
+            if (disk_rejects(c, T.lbnd[2 * first], T.lbnd[2 * first + 1], NORMAL_EPS)) count = 0;
             for (int j = 0; j < count; ++j) {
+                if (disk_rejects(c, T.tbnd[2 * (first + j)], T.tbnd[2 * (first + j) + 1], NORMAL_EPS)) continue;
                 f3 a, b, cc;
                 int32_t id;
                 load_tri(T.tris, first + j, a, b, cc, id);
@@ -611,51 +614,6 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
     }
 }
 
-// Packet form of avg_normal: every lane averages the normals of the faces within
-// squared distance NORMAL_EPS of its own point c.
-__device__ f3 avg_normal_packet(const Tree& T, f3 c, bool active) {
-    const int lane = (int)(threadIdx.x & 63);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    float cnt = 0.0f;
-    uint64_t mask = ballot(active);
-    WaveStack st;
-    int32_t ref = 0;
-    if (mask != 0) {
-        for (;;) {
-            const bool mine = (mask >> lane) & 1;
-            if (ref >= 0) {
-                UNode nd = load_unode(T.nodes, ref);
-                uint64_t m0 = ballot(mine && box_d2(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, c) < NORMAL_EPS);
-                uint64_t m1 = ballot(mine && box_d2(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, c) < NORMAL_EPS);
-                uint64_t m2 = ballot(mine && box_d2(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, c) < NORMAL_EPS);
-                uint64_t m3 = ballot(mine && box_d2(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, c) < NORMAL_EPS);
-                if (m0) st.push(nd.child.x, m0);
-                if (m1) st.push(nd.child.y, m1);
-                if (m2) st.push(nd.child.z, m2);
-                if (m3) st.push(nd.child.w, m3);
-            } else {
-                int32_t first, count;
-                leaf_range(ref, first, count);
-                for (int j = 0; j < count; ++j) {
-                    f3 a, b, cc;
-                    int32_t id;
-                    load_utri(T.tris, first + j, a, b, cc, id);
-                    if (mine && tri_dist2(a, b, cc, c) < NORMAL_EPS) {
-                        f3 n = tri_normal(a, b, cc);
-                        sx += (double)n.x;
-                        sy += (double)n.y;
-                        sz += (double)n.z;
-                        cnt += 1.0f;
-                    }
-                }
-            }
-            if (!st.pop(ref, mask)) break;
-            ref = uniform(ref);
-        }
-    }
-    return mk3((float)sx / cnt, (float)sy / cnt, (float)sz / cnt);
-}
-
 __device__ __forceinline__ f3 load_pt(const float* __restrict__ p, int64_t i) {
     return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
@@ -720,6 +678,38 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
     }
 }
 
+// Raystab sign of distances already computed (src/bvh.cu:408-427): lane i takes query
+// order[i] (or i) and negates distances[q] when all 64 stab rays from its point hit a
+// face.  The stab directions are keyed by the query index q, as in distance_kernel.
+template <int BLOCK, int LDS_N>
+__global__ __launch_bounds__(BLOCK) void stab_sign_kernel(int64_t n, const float* __restrict__ positions,
+                                                          const uint32_t* __restrict__ order,
+                                                          float* __restrict__ distances, Tree T) {
+    __shared__ int32_t s_ref[LDS_N * BLOCK];
+    __shared__ float s_key[LDS_N * BLOCK];
+    const int64_t gi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (gi >= n) return;
+    const int64_t q = order ? (int64_t)order[gi] : gi;
+    if (q >= n) return;
+    LaneStack<BLOCK, LDS_N> st;
+    st.ref = s_ref;
+    st.key = s_key;
+    st.tid = (int)threadIdx.x;
+    st.sp = 0;
+    const f3 p = load_pt(positions, q);
+    Pcg32 rng;
+    rng.advance((uint64_t)(uint32_t)((uint32_t)q * 2u));
+    float ox = rng.next_float();
+    float oy = rng.next_float();
+    for (uint32_t k = 0; k < 32; ++k) {
+        f3 dir = fibonacci_dir32(k, ox, oy);
+        if (!any_hit(T, p, neg3(dir), st) || !any_hit(T, p, dir, st)) return;  // a miss: outside
+    }
+    distances[q] = -distances[q];
+}
+
+constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal search
+
 // Packet kernel for the unsigned (MODE 0) and watertight signed (MODE 1) distance.
 // Lane i of the grid handles query order[i] (or i without a permutation).
 template <int MODE>
@@ -751,9 +741,20 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
     if (MODE == 1 || uvw) {
         if (valid) cp = tri_closest_pt(a, b, c, p);
     }
-    if (MODE == 1) {
-        f3 nrm = avg_normal_packet(T, cp, valid);
-        dist = copysignf(dist, dot3(nrm, sub3(p, cp)));
+    if constexpr (MODE == 1) {
+        // The normal search has a tiny radius around each lane's own closest point, so
+        // lanes walk it one by one (a packet would walk the union of 64 paths).
+        __shared__ int32_t s_ref[kNormalLds * 256];
+        __shared__ float s_key[kNormalLds * 256];
+        LaneStack<256, kNormalLds> st;
+        st.ref = s_ref;
+        st.key = s_key;
+        st.tid = (int)threadIdx.x;
+        st.sp = 0;
+        if (valid) {
+            f3 nrm = avg_normal(T, cp, st);
+            dist = copysignf(dist, dot3(nrm, sub3(p, cp)));
+        }
     }
     if (!valid) return;
     distances[q] = dist;
@@ -902,6 +903,16 @@ hipError_t launch_distance(const QueryArgs& q, int mode) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
     return launch_distance_impl(q, mode);
+}
+
+hipError_t launch_stab_sign(const QueryArgs& q) {
+    if (q.n <= 0) return hipSuccess;
+    if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
+    const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
+    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((dev::stab_sign_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)q.stream, q.n, q.in0, q.order, q.out_f, T);
+    return hipGetLastError();
 }
 
 hipError_t launch_ray(const QueryArgs& q) {

@@ -39,6 +39,8 @@ int stack_capacity_for(int need);
 // mode 0 unsigned distance, 1 signed (watertight), 2 signed (raystab)
 hipError_t launch_distance(const QueryArgs& q, int mode);
 hipError_t launch_ray(const QueryArgs& q);
+// Negates q.out_f[i] of every query whose 64 raystab rays all hit (distances from mode 0).
+hipError_t launch_stab_sign(const QueryArgs& q);
 
 // Morton permutation of point queries (order.hip).
 size_t morton_workspace_bytes(int64_t n);

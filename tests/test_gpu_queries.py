@@ -172,6 +172,11 @@ def test_sdf_raystab_bit_exact_closed_and_soup(gpu, oracle, ico4):
     np.testing.assert_array_equal(_np(fi), of)
     w = oracle.winding_f64(v, f, p)
     assert ((_np(d) < 0) == (w > 0.5))[np.abs(od) > 1e-4].all()
+    # below 2,048 queries the per-lane kernel runs instead of the ordered pair of kernels;
+    # the stab rays are keyed by the query index, so a prefix gives the same answers
+    ds, fs, _ = bvh.signed_distance(torch.from_numpy(p[:1000]).to(gpu), mode="raystab")
+    np.testing.assert_array_equal(_np(ds), od[:1000])
+    np.testing.assert_array_equal(_np(fs), of[:1000])
     # non-watertight soup: level-4 sphere minus 1% of its faces plus small random
     # triangles (with 1% holes about half of the inside points still see 64 hits)
     rng = np.random.default_rng(3)
