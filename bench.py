@@ -230,6 +230,28 @@ def run_extras(dev, steps, warmup, out):
     _, t = _time_kernel(lambda: bvh8.signed_distance(G, mode="watertight"), steps, 1)
     out["sdf_watertight_grid256"] = {"Mq_s": G.shape[0] / t / 1e6, "ms": t * 1e3}
 
+    # the same mesh built on the GPU (linear BVH): build time and UDF rate on that tree
+    V8 = torch.from_numpy(v8).to(dev)
+    F8 = torch.from_numpy(f8).to(dev)
+    lib = cubvh_amd._lib.load()
+    import ctypes
+
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    times = []
+    for _ in range(3):
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cubvh_amd._lib.check(lib.cubvh_bvh_create_gpu(dev.index, V8.data_ptr(), V8.shape[0], F8.data_ptr(),
+                                                      F8.shape[0], 0, ctypes.byref(h), stream))
+        times.append(time.perf_counter() - t0)
+        lib.cubvh_bvh_destroy(h)
+    gbvh = cubvh_amd.cuBVH(v8, f8, build="gpu")
+    _, t = _time_kernel(lambda: gbvh.unsigned_distance(G), steps, 1)
+    out["gpu_build_level8"] = {"build_ms": min(times) * 1e3, "udf_Mq_s": G.shape[0] / t / 1e6,
+                               "tree": gbvh.info()}
+    del gbvh, V8, F8
+
     # floodfill of the occupancy udf < 2/256
     d, _, _ = bvh8.unsigned_distance(G)
     occ = (d < 2.0 / 256).reshape(1, 256, 256, 256)

@@ -18,6 +18,10 @@ SIGNATURES = {
     "cubvh_last_error": (c_char_p, []),
     "cubvh_abi_version": (c_int, []),
     "cubvh_bvh_create": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int32, POINTER(c_void_p)]),
+    "cubvh_bvh_create_gpu": (
+        c_int,
+        [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int32, POINTER(c_void_p), c_void_p],
+    ),
     "cubvh_bvh_destroy": (None, [c_void_p]),
     "cubvh_bvh_info": (c_int, [c_void_p, c_void_p]),
     "cubvh_bvh_export": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -15,6 +15,7 @@
 
 #include "bvh.hpp"
 #include "forest.hpp"
+#include "gpubuild.hpp"
 #include "query.hpp"
 
 using namespace cubvh_amd;
@@ -76,24 +77,31 @@ struct DeviceTree {
     void* block = nullptr;
     TreeArrays view;
 
-    template <class H>
-    hipError_t upload(const H& h, hipStream_t stream) {
-        const void* src[6] = {h.nodes.data(), h.tris.data(), h.tbnd.data(), h.lbnd.data(), h.cbnd.data(), h.tpre.data()};
-        const size_t bytes[6] = {h.nodes.size() * sizeof(WideNode), h.tris.size() * sizeof(float),
-                                 h.tbnd.size() * sizeof(float),      h.lbnd.size() * sizeof(float),
-                                 h.cbnd.size() * sizeof(float),      h.tpre.size() * sizeof(float)};
-        size_t off[6], total = 0;
+    // Byte sizes of the six parts of a tree with n_nodes nodes and n_faces faces.
+    static void sizes(int64_t n_nodes, int64_t n_faces, size_t bytes[6]) {
+        bytes[0] = (size_t)n_nodes * sizeof(WideNode);
+        bytes[1] = 12 * (size_t)n_faces * sizeof(float);
+        bytes[2] = 8 * (size_t)n_faces * sizeof(float);
+        bytes[3] = 8 * (size_t)n_faces * sizeof(float);
+        bytes[4] = 32 * (size_t)n_nodes * sizeof(float);
+        bytes[5] = 36 * (size_t)n_faces * sizeof(float);
+    }
+    static size_t offsets(const size_t bytes[6], size_t off[6]) {
+        size_t total = 0;
         for (int i = 0; i < 6; ++i) {
             off[i] = total;
             total += (bytes[i] + 255) & ~(size_t)255;
         }
+        return total;
+    }
+
+    // Allocates the block for a tree of the given sizes and sets the view (no copy).
+    hipError_t reserve(int64_t n_nodes, int64_t n_faces) {
+        size_t bytes[6], off[6];
+        sizes(n_nodes, n_faces, bytes);
+        const size_t total = offsets(bytes, off);
         hipError_t e = hipMalloc(&block, std::max<size_t>(total, 256));
-        for (int i = 0; i < 6; ++i)
-            if (e == hipSuccess && bytes[i])
-                e = hipMemcpyAsync((char*)block + off[i], src[i], bytes[i], hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
-            (void)hipFree(block);
             block = nullptr;
             return e;
         }
@@ -104,6 +112,26 @@ struct DeviceTree {
         view.lbnd = (const float4*)(b + off[3]);
         view.cbnd = (const float4*)(b + off[4]);
         view.tpre = (const float4*)(b + off[5]);
+        return hipSuccess;
+    }
+
+    template <class H>
+    hipError_t upload(const H& h, hipStream_t stream) {
+        size_t bytes[6], off[6];
+        sizes((int64_t)h.nodes.size(), h.n_faces, bytes);
+        offsets(bytes, off);
+        const void* src[6] = {h.nodes.data(), h.tris.data(), h.tbnd.data(), h.lbnd.data(), h.cbnd.data(), h.tpre.data()};
+        hipError_t e = reserve((int64_t)h.nodes.size(), h.n_faces);
+        for (int i = 0; i < 6; ++i)
+            if (e == hipSuccess && bytes[i])
+                e = hipMemcpyAsync((char*)block + off[i], src[i], bytes[i], hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(block);
+            block = nullptr;
+            view = TreeArrays{};
+            return e;
+        }
         return hipSuccess;
     }
 };
@@ -122,13 +150,50 @@ struct cubvh_bvh {
         }
     }
 
+    // A tree built on the GPU lives only on that device (mirror_device) until a
+    // host-side consumer (state_dict, a forest, another device) needs it: then nodes
+    // and triangles are copied back and the derived arrays recomputed on the host.
+    int32_t mirror_device = -1;
+    int64_t n_nodes = 0;
+
+    int ensure_host_mirror_locked() {
+        if (mirror_device < 0) return CUBVH_OK;
+        DeviceScope g(mirror_device);
+        if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+        const TreeArrays& v = dev[mirror_device].view;
+        host.nodes.resize((size_t)n_nodes);
+        host.tris.resize(12 * (size_t)host.n_faces);
+        hipError_t e = hipMemcpy(host.nodes.data(), v.nodes, host.nodes.size() * sizeof(WideNode), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(host.tris.data(), v.tris, host.tris.size() * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail_hip(e, "copy of the GPU-built tree to the host");
+        mirror_device = -1;
+        return CUBVH_OK;
+    }
+    int ensure_host_derived_locked() {
+        int rc = ensure_host_mirror_locked();
+        if (rc != CUBVH_OK) return rc;
+        if (host.tpre.size() != 36 * (size_t)host.n_faces) compute_face_bounds(host);
+        return CUBVH_OK;
+    }
+    int ensure_host_mirror() {
+        std::lock_guard<std::mutex> lock(mu);
+        return ensure_host_mirror_locked();
+    }
+    int ensure_host_derived() {
+        std::lock_guard<std::mutex> lock(mu);
+        return ensure_host_derived_locked();
+    }
+
     // Uploads on first use (the reference's lazy upload, src/api_gpu.cu:76-78).
+
     int ensure_device(int device, hipStream_t stream) {
         if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
         std::lock_guard<std::mutex> lock(mu);
         if (dev[device].block) return CUBVH_OK;
         DeviceScope g(device);
         if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+        int rc = ensure_host_derived_locked();
+        if (rc != CUBVH_OK) return rc;
         hipError_t e = dev[device].upload(host, stream);
         if (e != hipSuccess) return fail_hip(e, "upload");
         return CUBVH_OK;
@@ -234,6 +299,7 @@ struct cubvh_forest {
 namespace {
 
 int finish_bvh(cubvh_bvh* b, cubvh_bvh** out) {
+    if (b->mirror_device < 0) b->n_nodes = (int64_t)b->host.nodes.size();
     if (stack_capacity_for(b->host.stack_need) < 0) {
         delete b;
         return fail(CUBVH_ERR_STATE, "tree too deep for the traversal stack");
@@ -352,11 +418,82 @@ int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* f
     CUBVH_GUARD_END
 }
 
+int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, const int32_t* faces,
+                         int64_t n_faces, int32_t leaf_size, cubvh_bvh** out, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!vertices || !faces) return fail(CUBVH_ERR_ARG, "vertices and faces must not be null");
+    if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "mesh must have vertices and faces");
+    if (n_faces > MAX_LEAF_FIRST || n_vertices > (int64_t)0x7fffffff)
+        return fail(CUBVH_ERR_ARG, "too many faces or vertices for the GPU build");
+    if (leaf_size < 0 || leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..8");
+    if (leaf_size == 0) leaf_size = kDefaultLeaf;
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    hipStream_t hs = (hipStream_t)stream;
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    if (n_faces <= leaf_size) {
+        // one leaf: nothing to build on the device; the host build over the copied-back mesh
+        std::vector<float> v(3 * (size_t)n_vertices);
+        std::vector<int32_t> fc(3 * (size_t)n_faces);
+        hipError_t e = hipMemcpyAsync(v.data(), vertices, v.size() * 4, hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess) e = hipMemcpyAsync(fc.data(), faces, fc.size() * 4, hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        if (e != hipSuccess) return fail_hip(e, "mesh copy");
+        return cubvh_bvh_create(v.data(), n_vertices, fc.data(), n_faces, leaf_size, out);
+    }
+    int rc = ensure_pool(device);
+    if (rc != CUBVH_OK) return rc;
+    void* ws = nullptr;
+    hipError_t e = hipMallocAsync(&ws, gpu_build_workspace_bytes(n_faces), hs);
+    if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(build workspace)");
+    struct WsGuard {
+        void* p;
+        hipStream_t s;
+        ~WsGuard() {
+            (void)hipFreeAsync(p, s);
+            (void)hipStreamSynchronize(s);
+        }
+    } guard{ws, hs};
+    const GpuBuildWorkspace w = carve_gpu_build_workspace(ws, n_faces);
+    GpuBuildResult res;
+    e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, w, res, hs);
+    if (e != hipSuccess) return fail_hip(e, "GPU tree build");
+    if (res.input_errors & 1) return fail(CUBVH_ERR_ARG, "a face references a vertex outside [0, n_vertices)");
+    if (res.input_errors & 2) return fail(CUBVH_ERR_ARG, "vertex coordinates must be finite");
+    if (stack_capacity_for(3 * res.depth + 1) < 0)
+        return fail(CUBVH_ERR_STATE, "GPU-built tree too deep for the traversal stack (depth " +
+                                         std::to_string(res.depth) + "); use the host build");
+    if (!res.structure_ok) return fail(CUBVH_ERR_STATE, "GPU-built tree failed its structure check");
+    cubvh_bvh* b = new cubvh_bvh;
+    HostBvh& h = b->host;
+    h.n_faces = n_faces;
+    h.leaf_size = leaf_size;
+    h.depth = res.depth;
+    h.stack_need = 3 * res.depth + 1;
+    b->n_nodes = res.n_nodes;
+    DeviceTree& dt = b->dev[device];
+    e = dt.reserve(res.n_nodes, n_faces);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(const_cast<WideNode*>(dt.view.nodes), w.wide, (size_t)res.n_nodes * sizeof(WideNode),
+                           hipMemcpyDeviceToDevice, hs);
+    if (e == hipSuccess) e = gpu_build_arrays(n_faces, w, dt.view, hs);
+    if (e == hipSuccess) e = gpu_build_face0_slot(w, h.face0_slot, hs);
+    if (e != hipSuccess) {
+        delete b;
+        return fail_hip(e, "GPU tree arrays");
+    }
+    b->mirror_device = device;  // nodes and triangles are copied to the host on demand
+    return finish_bvh(b, out);
+    CUBVH_GUARD_END
+}
+
 void cubvh_bvh_destroy(cubvh_bvh* bvh) { delete bvh; }
 
 int cubvh_bvh_info(const cubvh_bvh* bvh, int64_t info[5]) {
     if (!bvh || !info) return fail(CUBVH_ERR_ARG, "null argument");
-    info[0] = (int64_t)bvh->host.nodes.size();
+    info[0] = bvh->n_nodes;
     info[1] = bvh->host.n_faces;
     info[2] = bvh->host.depth;
     info[3] = bvh->host.stack_need;
@@ -366,6 +503,8 @@ int cubvh_bvh_info(const cubvh_bvh* bvh, int64_t info[5]) {
 
 int cubvh_bvh_export(const cubvh_bvh* bvh, int32_t* nodes, int32_t* triangles) {
     if (!bvh || !nodes || !triangles) return fail(CUBVH_ERR_ARG, "null argument");
+    int rc = const_cast<cubvh_bvh*>(bvh)->ensure_host_mirror();  // a cache: logically const
+    if (rc != CUBVH_OK) return rc;
     std::memcpy(nodes, bvh->host.nodes.data(), bvh->host.nodes.size() * sizeof(WideNode));
     for (int64_t s = 0; s < bvh->host.n_faces; ++s) export_record(&bvh->host.tris[12 * s], triangles + 12 * s);
     return CUBVH_OK;
@@ -464,6 +603,8 @@ int cubvh_bvh_import_reference(const int32_t* nodes9, int64_t n_nodes, const int
 
 int cubvh_bvh_triangles(const cubvh_bvh* bvh, float* tri_vertices) {
     if (!bvh || !tri_vertices) return fail(CUBVH_ERR_ARG, "null argument");
+    int rc = const_cast<cubvh_bvh*>(bvh)->ensure_host_mirror();  // a cache: logically const
+    if (rc != CUBVH_OK) return rc;
     const HostBvh& h = bvh->host;
     for (int64_t s = 0; s < h.n_faces; ++s) {
         const float* in = &h.tris[12 * s];
@@ -615,6 +756,11 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
         if (!bvhs[m]) {
             delete f;
             return fail(CUBVH_ERR_ARG, "null BVH in the list");
+        }
+        int rc = bvhs[m]->ensure_host_derived();
+        if (rc != CUBVH_OK) {
+            delete f;
+            return rc;
         }
         const HostBvh& h = bvhs[m]->host;
         const int64_t node_base = (int64_t)f->nodes.size();

@@ -1,0 +1,638 @@
+// gpubuild.hip -- construction of the 4-wide BVH on the GPU.
+//
+// Builds the same tree layout as the host builder (build.cpp, which replaces the
+// reference's src/bvh.cu:515-620) from a mesh already in device memory:
+//   1. faces    : gather each face's vertices, check indices and coordinates, bounds
+//                 of the centroids and the largest |coordinate| (order-preserving
+//                 int atomics);
+//   2. keys     : 63-bit Morton code of each centroid (21 bits per axis) and a radix
+//                 sort of (code, face) pairs;
+//   3. tree     : the binary radix tree over the sorted codes (Karras, "Maximizing
+//                 parallelism in the construction of BVHs, octrees and k-d trees",
+//                 HPG 2012), equal codes ordered by position; one thread per
+//                 internal node;
+//   4. boxes    : bottom-up from the leaves; of the two threads reaching a node the
+//                 second computes its box (acquire-release counter per node);
+//   5. collapse : top-down, one launch per level: a wide node takes its binary
+//                 node's two children and opens the largest-area child until it has
+//                 4; subtrees of <= leaf_size faces become leaves;
+//   6. arrays   : leaf-ordered triangles, face / leaf / subtree bounding disks and
+//                 the per-face constants of the distance formula, with the layouts
+//                 and rules of the host build (bvh.hpp).
+// A subtree of the radix tree covers a contiguous run of sorted faces, so leaves
+// and subtree disks are ranges of the leaf order.  Child boxes are widened by the
+// host build's margin, so query results are the same whichever builder made the tree.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "gpubuild.hpp"
+
+namespace cubvh_amd {
+namespace gb {
+
+__device__ __forceinline__ int32_t ordered(float f) {
+    int32_t i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float unordered(int32_t i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// the counter layout is listed at GpuBuildWorkspace::counters
+__global__ void init_counters(int32_t* c) {
+    if (threadIdx.x < 3) c[threadIdx.x] = 0x7fffffff;                  // centroid lo (ordered)
+    else if (threadIdx.x < 6) c[threadIdx.x] = (int32_t)0x80000000;    // centroid hi (ordered)
+    else if (threadIdx.x < 16) c[threadIdx.x] = 0;                     // the rest
+}
+
+// Gathers face f's vertices into tri9[9f..], checks them, and folds its centroid and
+// |coordinates| into the counters.
+__global__ __launch_bounds__(256) void gather_faces(int64_t n, const float* __restrict__ V, int64_t nv,
+                                                    const int32_t* __restrict__ F, float* __restrict__ tri9,
+                                                    int32_t* c) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float amax = 0.0f;
+    int32_t err = 0;
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n; f += (int64_t)gridDim.x * blockDim.x) {
+        float t[9];
+        for (int k = 0; k < 3; ++k) {
+            const int32_t vi = F[3 * f + k];
+            if (vi < 0 || (int64_t)vi >= nv) {
+                err |= 1;
+                t[3 * k] = t[3 * k + 1] = t[3 * k + 2] = 0.0f;
+                continue;
+            }
+            for (int a = 0; a < 3; ++a) t[3 * k + a] = V[3 * (int64_t)vi + a];
+        }
+        for (int k = 0; k < 9; ++k) {
+            if (!isfinite(t[k])) err |= 2;
+            amax = fmaxf(amax, fabsf(t[k]));
+            tri9[9 * f + k] = t[k];
+        }
+        for (int a = 0; a < 3; ++a) {
+            const float cen = (t[a] + t[3 + a] + t[6 + a]) * (1.0f / 3.0f);
+            lo[a] = fminf(lo[a], cen);
+            hi[a] = fmaxf(hi[a], cen);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (lo[a] <= hi[a]) {
+            atomicMin(&c[a], ordered(lo[a]));
+            atomicMax(&c[3 + a], ordered(hi[a]));
+        }
+    }
+    if (amax > 0.0f) atomicMax(&c[6], __float_as_int(amax));  // non-negative: int order = float order
+    if (err) atomicOr(&c[7], err);
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void morton_keys(int64_t n, const float* __restrict__ tri9,
+                                                   const int32_t* __restrict__ c, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    uint64_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+        const float lo = unordered(c[a]), hi = unordered(c[3 + a]);
+        const float cen = (tri9[9 * f + a] + tri9[9 * f + 3 + a] + tri9[9 * f + 6 + a]) * (1.0f / 3.0f);
+        const float ext = hi - lo;
+        float u = ext > 0.0f ? (cen - lo) / ext : 0.0f;
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        const uint64_t q = (uint64_t)fminf(u * 2097152.0f, 2097151.0f);
+        code |= spread21(q) << (2 - a);
+    }
+    keys[f] = code;
+    vals[f] = (uint32_t)f;
+}
+
+// Common-prefix length of sorted keys i and j (-1 outside [0, n)); equal keys are
+// told apart by their positions.
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clzll((long long)((uint64_t)i ^ (uint64_t)j));
+}
+
+// Internal node i of the radix tree over n sorted keys (n >= 2).  Children: >= 0
+// internal node, < 0 leaf ~j (sorted position j).
+__global__ __launch_bounds__(256) void radix_tree(int64_t n, const uint64_t* __restrict__ k, BinNode* __restrict__ bn,
+                                                  int32_t* __restrict__ leaf_parent) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = delta(k, n, i, i + 1) - delta(k, n, i, i - 1) > 0 ? 1 : -1;
+    const int dmin = delta(k, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(k, n, i, j);
+    int64_t s = 0;
+    for (int64_t div = 2;; div *= 2) {
+        const int64_t t = (l + div - 1) / div;
+        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    BinNode& b = bn[i];
+    b.first = (int32_t)lo;
+    b.last = (int32_t)hi;
+    b.left = lo == g ? ~(int32_t)g : (int32_t)g;
+    b.right = hi == g + 1 ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
+    if (b.left < 0) leaf_parent[g] = (int32_t)i;
+    else bn[g].parent = (int32_t)i;
+    if (b.right < 0) leaf_parent[g + 1] = (int32_t)i;
+    else bn[g + 1].parent = (int32_t)i;
+    if (i == 0) b.parent = -1;
+}
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ Box face_box(const float* __restrict__ tri9, int64_t f) {
+    Box r;
+    for (int a = 0; a < 3; ++a) {
+        const float x = tri9[9 * f + a], y = tri9[9 * f + 3 + a], z = tri9[9 * f + 6 + a];
+        r.lo[a] = fminf(x, fminf(y, z));
+        r.hi[a] = fmaxf(x, fmaxf(y, z));
+    }
+    return r;
+}
+
+__device__ __forceinline__ Box node_box(const BinNode& b) {
+    Box r;
+    for (int a = 0; a < 3; ++a) {
+        r.lo[a] = b.lo[a];
+        r.hi[a] = b.hi[a];
+    }
+    return r;
+}
+
+__device__ __forceinline__ Box child_box(const BinNode* bn, const uint32_t* __restrict__ face_of,
+                                         const float* __restrict__ tri9, int32_t ref) {
+    return ref < 0 ? face_box(tri9, face_of[~ref]) : node_box(bn[ref]);
+}
+
+// One thread per leaf walks up; the second arrival at a node computes its box.
+__global__ __launch_bounds__(256) void boxes_up(int64_t n, const uint32_t* __restrict__ face_of,
+                                                const float* __restrict__ tri9, BinNode* bn,
+                                                const int32_t* __restrict__ leaf_parent, int32_t* arrivals) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int32_t p = leaf_parent[j];
+    while (p >= 0) {
+        const int32_t before =
+            __hip_atomic_fetch_add(&arrivals[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (before == 0) return;  // the sibling subtree is not done yet: it continues
+        const Box a = child_box(bn, face_of, tri9, __hip_atomic_load(&bn[p].left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const Box b = child_box(bn, face_of, tri9, __hip_atomic_load(&bn[p].right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int k = 0; k < 3; ++k) {
+            bn[p].lo[k] = fminf(a.lo[k], b.lo[k]);
+            bn[p].hi[k] = fmaxf(a.hi[k], b.hi[k]);
+        }
+        p = bn[p].parent;
+    }
+}
+
+// Outward rounding by a relative and an absolute margin (the host build's rule).
+__device__ __forceinline__ float pad_down(float v, double abs_pad) {
+    return __double2float_rd((double)v - fabs((double)v) * 1.0e-6 - abs_pad);
+}
+__device__ __forceinline__ float pad_up(float v, double abs_pad) {
+    return __double2float_ru((double)v + fabs((double)v) * 1.0e-6 + abs_pad);
+}
+
+__device__ __forceinline__ float box_area(const Box& b) {
+    const float d0 = b.hi[0] - b.lo[0], d1 = b.hi[1] - b.lo[1], d2 = b.hi[2] - b.lo[2];
+    return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
+}
+
+// One wide node per thread: entry (binary node, wide node, depth).
+__global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, int32_t n_in, int4* __restrict__ out,
+                                                      const BinNode* __restrict__ bn,
+                                                      const uint32_t* __restrict__ face_of,
+                                                      const float* __restrict__ tri9, int32_t leaf_size,
+                                                      WideNode* __restrict__ wn, int2* __restrict__ leaves,
+                                                      int4* __restrict__ jobs, int32_t* c) {
+    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n_in) return;
+    const int4 ent = in[e];
+    const BinNode& root = bn[ent.x];
+    int32_t kids[4] = {root.left, root.right, 0, 0};
+    int nk = 2;
+    auto size_of = [&](int32_t r) -> int32_t { return r < 0 ? 1 : bn[r].last - bn[r].first + 1; };
+    while (nk < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int q = 0; q < nk; ++q) {
+            const int32_t r = kids[q];
+            if (r >= 0 && size_of(r) > leaf_size) {
+                const float a = box_area(node_box(bn[r]));
+                if (a > ba) {
+                    ba = a;
+                    best = q;
+                }
+            }
+        }
+        if (best < 0) break;
+        const int32_t o = kids[best];
+        for (int q = nk; q > best + 1; --q) kids[q] = kids[q - 1];
+        kids[best] = bn[o].left;
+        kids[best + 1] = bn[o].right;
+        ++nk;
+    }
+    const double abs_pad = (double)(fmaxf(__int_as_float(c[6]), 1e-30f) * 4.0e-6f);
+    WideNode w;
+    for (int q = 0; q < 4; ++q) {
+        w.lo_x[q] = w.lo_y[q] = w.lo_z[q] = INFINITY;
+        w.hi_x[q] = w.hi_y[q] = w.hi_z[q] = -INFINITY;
+        w.child[q] = CHILD_EMPTY;
+    }
+    w.n_children = nk;
+    w.pad[0] = w.pad[1] = w.pad[2] = 0;
+    for (int q = 0; q < nk; ++q) {
+        const int32_t r = kids[q];
+        const Box b = child_box(bn, face_of, tri9, r);
+        w.lo_x[q] = pad_down(b.lo[0], abs_pad);
+        w.lo_y[q] = pad_down(b.lo[1], abs_pad);
+        w.lo_z[q] = pad_down(b.lo[2], abs_pad);
+        w.hi_x[q] = pad_up(b.hi[0], abs_pad);
+        w.hi_y[q] = pad_up(b.hi[1], abs_pad);
+        w.hi_z[q] = pad_up(b.hi[2], abs_pad);
+        const int32_t first = r < 0 ? ~r : bn[r].first;
+        const int32_t cnt = size_of(r);
+        if (cnt <= leaf_size) {
+            w.child[q] = (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(cnt - 1));  // leaf reference
+            leaves[atomicAdd(&c[10], 1)] = make_int2(first, cnt);
+        } else {
+            const int32_t id = atomicAdd(&c[8], 1);
+            w.child[q] = id;
+            out[atomicAdd(&c[9], 1)] = make_int4(r, id, ent.z + 1, 0);
+            jobs[atomicAdd(&c[11], 1)] = make_int4(ent.y, q, r, 0);
+        }
+    }
+    wn[ent.y] = w;
+    atomicMax(&c[12], ent.z);
+}
+
+// Leaf-ordered triangles (bvh.hpp) and the slot of original face 0.
+__global__ __launch_bounds__(256) void emit_tris(int64_t n, const uint32_t* __restrict__ face_of,
+                                                 const float* __restrict__ tri9, float4* __restrict__ tris,
+                                                 int32_t* c) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t f = face_of[s];
+    const float* t = tri9 + 9 * (int64_t)f;
+    tris[3 * s + 0] = make_float4(t[0], t[1], t[2], __int_as_float((int32_t)f));
+    tris[3 * s + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+    tris[3 * s + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+    if (f == 0) c[13] = (int32_t)s;
+}
+
+// Bounding disk of the vertices of slots [first, first + count): centre of their box,
+// area-weighted mean normal, radius and thickness measured in double from the rounded
+// centre along the rounded normal and rounded outward (the host build's rule).
+struct DiskAcc {
+    double lo[3], hi[3], n[3];
+};
+
+__device__ __forceinline__ void acc_init(DiskAcc& a) {
+    for (int k = 0; k < 3; ++k) {
+        a.lo[k] = INFINITY;
+        a.hi[k] = -INFINITY;
+        a.n[k] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void acc_face(DiskAcc& a, const float4* __restrict__ tris, int64_t s) {
+    double v[3][3];
+    for (int i = 0; i < 3; ++i) {
+        const float4 r = tris[3 * s + i];
+        v[i][0] = r.x;
+        v[i][1] = r.y;
+        v[i][2] = r.z;
+    }
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = v[1][k] - v[0][k];
+        e2[k] = v[2][k] - v[0][k];
+        for (int i = 0; i < 3; ++i) {
+            a.lo[k] = fmin(a.lo[k], v[i][k]);
+            a.hi[k] = fmax(a.hi[k], v[i][k]);
+        }
+    }
+    a.n[0] += e1[1] * e2[2] - e1[2] * e2[1];
+    a.n[1] += e1[2] * e2[0] - e1[0] * e2[2];
+    a.n[2] += e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+struct DiskFrame {
+    float c[3], n[3];
+};
+
+__device__ __forceinline__ DiskFrame frame_of(const double* centre, const double* dir) {
+    DiskFrame f;
+    for (int k = 0; k < 3; ++k) {
+        f.c[k] = (float)centre[k];
+        f.n[k] = 0.0f;
+    }
+    const double nl = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (nl > 0.0 && isfinite(nl))
+        for (int k = 0; k < 3; ++k) f.n[k] = (float)(dir[k] / nl);
+    return f;
+}
+
+// Largest in-plane distance (x) and height (y) of slot s's vertices in frame f.
+__device__ __forceinline__ double2 face_extent(const DiskFrame& f, const float4* __restrict__ tris, int64_t s) {
+    double r = 0.0, th = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const float4 v = tris[3 * s + i];
+        const double d[3] = {(double)v.x - f.c[0], (double)v.y - f.c[1], (double)v.z - f.c[2]};
+        const double h = d[0] * f.n[0] + d[1] * f.n[1] + d[2] * f.n[2];
+        const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        r = fmax(r, sqrt(fmax(dd - h * h, 0.0)));
+        th = fmax(th, fabs(h));
+    }
+    return make_double2(r, th);
+}
+
+__device__ __forceinline__ void store_disk(const DiskFrame& f, double r, double th, float4* __restrict__ o) {
+    r = r * (1.0 + 1e-6) + 1e-30;
+    th = th * (1.0 + 1e-6) + 1e-30;
+    o[0] = make_float4(f.c[0], f.c[1], f.c[2], __double2float_ru(r));
+    o[1] = make_float4(f.n[0], f.n[1], f.n[2], __double2float_ru(th));
+}
+
+// fp32 operations of the distance formula, with its fused operations only.
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float vdot(V3 a, V3 b) {
+    return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x));
+}
+__device__ __forceinline__ V3 vcross(V3 a, V3 b) {
+    return V3{__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+              __builtin_fmaf(a.x, b.y, -(a.y * b.x))};
+}
+
+// Per face: bounding disk (centroid, plane normal) and the distance-formula constants.
+__global__ __launch_bounds__(256) void face_records(int64_t n, const float4* __restrict__ tris,
+                                                    float4* __restrict__ tbnd, float4* __restrict__ tpre) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    DiskAcc acc;
+    acc_init(acc);
+    acc_face(acc, tris, s);
+    double cen[3];
+    for (int k = 0; k < 3; ++k) {
+        double sum = 0.0;
+        for (int i = 0; i < 3; ++i) sum += (double)((const float*)&tris[3 * s + i])[k];
+        cen[k] = sum / 3.0;
+    }
+    const DiskFrame fr = frame_of(cen, acc.n);
+    const double2 ext = face_extent(fr, tris, s);
+    store_disk(fr, ext.x, ext.y, tbnd + 2 * s);
+    const float4 r0 = tris[3 * s], r1 = tris[3 * s + 1], r2 = tris[3 * s + 2];
+    const V3 a{r0.x, r0.y, r0.z}, b{r1.x, r1.y, r1.z}, c{r2.x, r2.y, r2.z};
+    const V3 v21 = vsub(b, a), v32 = vsub(c, b), v13 = vsub(a, c);
+    const V3 nn = vcross(v21, v13);
+    const V3 g1 = vcross(v21, nn), g2 = vcross(v32, nn), g3 = vcross(v13, nn);
+    float4* o = tpre + 9 * s;
+    o[0] = make_float4(a.x, a.y, a.z, r0.w);
+    o[1] = make_float4(b.x, b.y, b.z, vdot(v21, v21));
+    o[2] = make_float4(c.x, c.y, c.z, vdot(v32, v32));
+    o[3] = make_float4(v21.x, v21.y, v21.z, vdot(v13, v13));
+    o[4] = make_float4(v32.x, v32.y, v32.z, vdot(nn, nn));
+    o[5] = make_float4(v13.x, v13.y, v13.z, nn.x);
+    o[6] = make_float4(g1.x, g1.y, g1.z, nn.y);
+    o[7] = make_float4(g2.x, g2.y, g2.z, nn.z);
+    o[8] = make_float4(g3.x, g3.y, g3.z, 0.0f);
+}
+
+// One thread per leaf (first, count): the leaf's disk at lbnd[2 * first].
+__global__ __launch_bounds__(256) void leaf_records(const int2* __restrict__ leaves, const int32_t* c,
+                                                    const float4* __restrict__ tris, float4* __restrict__ lbnd) {
+    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= c[10]) return;
+    const int2 lf = leaves[e];
+    DiskAcc acc;
+    acc_init(acc);
+    for (int j = 0; j < lf.y; ++j) acc_face(acc, tris, lf.x + j);
+    double cen[3];
+    for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (acc.lo[k] + acc.hi[k]);
+    const DiskFrame fr = frame_of(cen, acc.n);
+    double r = 0.0, th = 0.0;
+    for (int j = 0; j < lf.y; ++j) {
+        const double2 x = face_extent(fr, tris, lf.x + j);
+        r = fmax(r, x.x);
+        th = fmax(th, x.y);
+    }
+    store_disk(fr, r, th, lbnd + 2 * lf.x);
+}
+
+// Structural check of the collapsed tree: every sorted face lies in exactly one leaf.
+__global__ __launch_bounds__(256) void cover_leaves(const int2* __restrict__ leaves, const int32_t* c,
+                                                    int32_t* cover) {
+    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= c[10]) return;
+    const int2 lf = leaves[e];
+    for (int j = 0; j < lf.y; ++j) atomicAdd(&cover[lf.x + j], 1);
+}
+__global__ __launch_bounds__(256) void check_cover(int64_t n, const int32_t* __restrict__ cover, int32_t* c) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && cover[s] != 1) atomicOr(&c[14], 1);
+}
+
+constexpr int kJobThreads = 256;
+
+__device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, bool is_max) {
+    // sh: kJobThreads * nv doubles
+    for (int k = 0; k < nv; ++k) sh[k * kJobThreads + threadIdx.x] = v[k];
+    __syncthreads();
+    for (int step = kJobThreads / 2; step > 0; step >>= 1) {
+        if ((int)threadIdx.x < step)
+            for (int k = 0; k < nv; ++k) {
+                const double a = sh[k * kJobThreads + threadIdx.x], b = sh[k * kJobThreads + threadIdx.x + step];
+                sh[k * kJobThreads + threadIdx.x] = is_max ? fmax(a, b) : a + b;
+            }
+        __syncthreads();
+    }
+    for (int k = 0; k < nv; ++k) v[k] = sh[k * kJobThreads];
+    __syncthreads();
+}
+
+// One block per internal child (wide node, slot, binary node): the disk of every
+// face of the binary node's range, at cbnd[8 * wide + 2 * slot].
+__global__ __launch_bounds__(kJobThreads) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
+                                                               const BinNode* __restrict__ bn,
+                                                               const float4* __restrict__ tris,
+                                                               float4* __restrict__ cbnd) {
+    __shared__ double sh[3 * kJobThreads];
+    for (int32_t e = blockIdx.x; e < c[11]; e += gridDim.x) {
+        const int4 job = jobs[e];
+        const BinNode& b = bn[job.z];
+        double nsum[3] = {0.0, 0.0, 0.0};
+        for (int64_t s = b.first + (int64_t)threadIdx.x; s <= b.last; s += kJobThreads) {
+            DiskAcc acc;
+            acc_init(acc);
+            acc_face(acc, tris, s);
+            for (int k = 0; k < 3; ++k) nsum[k] += acc.n[k];
+        }
+        block_reduce(sh, nsum, 3, false);
+        double cen[3];
+        for (int k = 0; k < 3; ++k) cen[k] = 0.5 * ((double)b.lo[k] + (double)b.hi[k]);
+        const DiskFrame fr = frame_of(cen, nsum);
+        double ext[2] = {0.0, 0.0};
+        for (int64_t s = b.first + (int64_t)threadIdx.x; s <= b.last; s += kJobThreads) {
+            const double2 x = face_extent(fr, tris, s);
+            ext[0] = fmax(ext[0], x.x);
+            ext[1] = fmax(ext[1], x.y);
+        }
+        block_reduce(sh, ext, 2, true);
+        if (threadIdx.x == 0) store_disk(fr, ext[0], ext[1], cbnd + 8 * (int64_t)job.x + 2 * job.y);
+    }
+}
+
+}  // namespace gb
+
+size_t gpu_build_workspace_bytes(int64_t n) {
+    size_t sort_tmp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const int64_t ni = n > 1 ? n - 1 : 1;
+    return al(64 * sizeof(int32_t)) + al(9 * n * sizeof(float)) + 2 * al(n * sizeof(uint64_t)) +
+           2 * al(n * sizeof(uint32_t)) + al(sort_tmp) + al(ni * sizeof(BinNode)) + al(n * sizeof(int32_t)) +
+           al(ni * sizeof(int32_t)) + al(ni * sizeof(WideNode)) + 2 * al(ni * sizeof(int4)) +
+           al(n * sizeof(int2)) + al(ni * sizeof(int4));
+}
+
+GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
+    size_t sort_tmp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const int64_t ni = n > 1 ? n - 1 : 1;
+    char* p = (char*)ws;
+    GpuBuildWorkspace w;
+    w.counters = (int32_t*)p; p += al(64 * sizeof(int32_t));
+    w.tri9 = (float*)p; p += al(9 * n * sizeof(float));
+    w.keys0 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
+    w.keys1 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
+    w.vals0 = (uint32_t*)p; p += al(n * sizeof(uint32_t));
+    w.face_of = (uint32_t*)p; p += al(n * sizeof(uint32_t));
+    w.sort_tmp = p; w.sort_tmp_bytes = sort_tmp; p += al(sort_tmp);
+    w.bn = (BinNode*)p; p += al(ni * sizeof(BinNode));
+    w.leaf_parent = (int32_t*)p; p += al(n * sizeof(int32_t));
+    w.arrivals = (int32_t*)p; p += al(ni * sizeof(int32_t));
+    w.wide = (WideNode*)p; p += al(ni * sizeof(WideNode));
+    w.queue0 = (int4*)p; p += al(ni * sizeof(int4));
+    w.queue1 = (int4*)p; p += al(ni * sizeof(int4));
+    w.leaves = (int2*)p; p += al(n * sizeof(int2));
+    w.jobs = (int4*)p; p += al(ni * sizeof(int4));
+    w.n = n;
+    return w;
+}
+
+namespace {
+unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+}  // namespace
+
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
+                          const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    hipLaunchKernelGGL(gb::init_counters, dim3(1), dim3(64), 0, s, w.counters);
+    hipLaunchKernelGGL(gb::gather_faces, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, n, V, nv, F,
+                       w.tri9, w.counters);
+    int32_t c[16];
+    e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    res.input_errors = c[7];
+    if (c[7]) return hipSuccess;
+    hipLaunchKernelGGL(gb::morton_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.counters, w.keys0,
+                       w.vals0);
+    size_t tmp = w.sort_tmp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp, w.keys0, w.keys1, w.vals0, w.face_of, (int)n, 0, 63, s);
+    if (e != hipSuccess) return e;
+    const int64_t ni = n - 1;
+    hipLaunchKernelGGL(gb::radix_tree, dim3(blocks_for(ni)), dim3(256), 0, s, n, w.keys1, w.bn, w.leaf_parent);
+    hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(ni), 4096)), dim3(256), 0, s, w.arrivals, ni, 0);
+    hipLaunchKernelGGL(gb::boxes_up, dim3(blocks_for(n)), dim3(256), 0, s, n, w.face_of, w.tri9, w.bn, w.leaf_parent,
+                       w.arrivals);
+    // collapse, one level per launch; the root entry is (binary root 0, wide node 0, depth 1)
+    const int4 root = make_int4(0, 0, 1, 0);
+    const int32_t one = 1;
+    e = hipMemcpyAsync(w.queue0, &root, sizeof root, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(w.counters + 8, &one, sizeof one, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    int4* qin = w.queue0;
+    int4* qout = w.queue1;
+    int32_t n_in = 1;
+    for (int level = 0; n_in > 0; ++level) {
+        if (level > 64) return hipErrorInvalidValue;  // deeper than any stack accepts
+        const int32_t zero = 0;
+        e = hipMemcpyAsync(w.counters + 9, &zero, sizeof zero, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(gb::collapse_level, dim3(blocks_for(n_in)), dim3(256), 0, s, qin, n_in, qout, w.bn,
+                           w.face_of, w.tri9, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+        e = hipMemcpyAsync(&n_in, w.counters + 9, sizeof n_in, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        std::swap(qin, qout);
+    }
+    // every sorted face in exactly one leaf (leaf_parent is free again: reused as counts)
+    hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, w.leaf_parent, n,
+                       0);
+    hipLaunchKernelGGL(gb::cover_leaves, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, w.leaf_parent);
+    hipLaunchKernelGGL(gb::check_cover, dim3(blocks_for(n)), dim3(256), 0, s, n, w.leaf_parent, w.counters);
+    e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    res.structure_ok = c[14] == 0;
+    res.n_nodes = c[8];
+    res.n_leaves = c[10];
+    res.n_jobs = c[11];
+    res.depth = c[12];
+    return hipGetLastError();
+}
+
+hipError_t gpu_build_face0_slot(const GpuBuildWorkspace& w, int64_t& slot, hipStream_t s) {
+    int32_t v = -1;
+    hipError_t e = hipMemcpyAsync(&v, w.counters + 13, sizeof v, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    slot = v;
+    return e;
+}
+
+hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArrays& out, hipStream_t s) {
+    float4* tris = const_cast<float4*>(out.tris);
+    hipLaunchKernelGGL(gb::emit_tris, dim3(blocks_for(n)), dim3(256), 0, s, n, w.face_of, w.tri9, tris, w.counters);
+    hipLaunchKernelGGL(gb::face_records, dim3(blocks_for(n)), dim3(256), 0, s, n, out.tris,
+                       const_cast<float4*>(out.tbnd), const_cast<float4*>(out.tpre));
+    hipLaunchKernelGGL(gb::leaf_records, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, out.tris,
+                       const_cast<float4*>(out.lbnd));
+    hipLaunchKernelGGL(gb::subtree_records, dim3(2048), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters, w.bn,
+                       out.tris, const_cast<float4*>(out.cbnd));
+    return hipGetLastError();
+}
+
+}  // namespace cubvh_amd

@@ -1,0 +1,63 @@
+// gpubuild.hpp -- host interface of the GPU tree build (gpubuild.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh.hpp"
+#include "query.hpp"
+
+namespace cubvh_amd {
+
+// Binary radix-tree node: children >= 0 are internal nodes, < 0 the leaf ~j (sorted
+// face j); [first, last] is the node's run of sorted faces; box of its vertices.
+struct BinNode {
+    int32_t left, right, first, last, parent;
+    float lo[3], hi[3];
+};
+
+struct GpuBuildWorkspace {
+    int32_t* counters = nullptr;  // [0..5] centroid bounds (ordered ints), [6] max |coordinate| bits,
+                                  // [7] input errors (1 bad index, 2 non-finite), [8] wide nodes,
+                                  // [9] next queue length, [10] leaves, [11] subtree jobs,
+                                  // [12] depth, [13] slot of face 0, [14] structure error
+    float* tri9 = nullptr;        // [9n] gathered vertices, original face order
+    uint64_t* keys0 = nullptr;
+    uint64_t* keys1 = nullptr;    // sorted Morton codes
+    uint32_t* vals0 = nullptr;
+    uint32_t* face_of = nullptr;  // [n] original face of sorted position (= leaf slot)
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    BinNode* bn = nullptr;        // [n-1]
+    int32_t* leaf_parent = nullptr;
+    int32_t* arrivals = nullptr;
+    WideNode* wide = nullptr;     // [n-1] wide nodes (the first `n_nodes` used)
+    int4* queue0 = nullptr;
+    int4* queue1 = nullptr;
+    int2* leaves = nullptr;       // (first, count) per leaf
+    int4* jobs = nullptr;         // (wide node, slot, binary node) per internal child
+    int64_t n = 0;
+};
+
+struct GpuBuildResult {
+    int32_t input_errors = 0;
+    int32_t n_nodes = 0, n_leaves = 0, n_jobs = 0, depth = 0;
+    bool structure_ok = false;  // every face in exactly one leaf
+};
+
+size_t gpu_build_workspace_bytes(int64_t n);
+GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n);
+
+// Steps 1-5 (n >= 2 faces): wide nodes in w.wide, leaf order in w.face_of.  Returns
+// with res.input_errors set (and nothing built) when an index or coordinate is bad.
+// Synchronizes the stream once per tree level.
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
+                          const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s);
+
+// Step 6: triangles and derived arrays into `out` (nodes are copied by the caller).
+hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArrays& out, hipStream_t s);
+
+// Leaf-order slot of original face 0 (after gpu_build_arrays).  Synchronizes.
+hipError_t gpu_build_face0_slot(const GpuBuildWorkspace& w, int64_t& slot, hipStream_t s);
+
+}  // namespace cubvh_amd

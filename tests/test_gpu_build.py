@@ -1,0 +1,126 @@
+"""Tree built on the GPU (cubvh_bvh_create_gpu): structure, and query results
+identical to the host-built tree and to the oracle.
+
+Query results do not depend on the tree (canonical ties to the lowest face index and
+conservative box tests), so a GPU-built tree must give bit-identical answers.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import cubvh_amd
+from cubvh_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _leaves_cover_faces(bvh, v, f):
+    sd = bvh.state_dict()
+    nodes = sd["nodes"].numpy()
+    rec = sd["triangles"].numpy()
+    abc = rec[:, :9].view(np.float32).reshape(-1, 3, 3)
+    ids = rec.view(np.int64)[:, 5]
+    assert sorted(ids.tolist()) == list(range(len(f)))
+    np.testing.assert_array_equal(abc, v[f[ids]])
+    bounds = nodes[:, :24].view(np.float32).reshape(-1, 6, 4)
+    seen = np.zeros(len(f), np.int64)
+    for ni in range(len(nodes)):
+        for k in range(4):
+            r = nodes[ni, 24 + k]
+            if r == -1 or r >= 0:
+                continue
+            u = np.uint32(r & 0x7FFFFFFF)
+            first, cnt = int(u >> 3), int(u & 7) + 1
+            seen[first:first + cnt] += 1
+            tri = abc[first:first + cnt]
+            assert (tri.min(axis=(0, 1)) >= bounds[ni, 0::2, k]).all()
+            assert (tri.max(axis=(0, 1)) <= bounds[ni, 1::2, k]).all()
+    assert (seen == 1).all()
+
+
+@pytest.mark.parametrize("leaf", [0, 1, 4])
+def test_gpu_build_structure(gpu, leaf):
+    v, f, _ = S.perturbed_sphere(5, seed=2)
+    bvh = cubvh_amd.cuBVH(v, f, leaf_size=leaf, build="gpu")
+    info = bvh.info()
+    assert info["n_faces"] == len(f) and info["stack_need"] == 3 * info["depth"] + 1 <= 64
+    assert info["leaf_size"] == (leaf or 8)
+    _leaves_cover_faces(bvh, v, f)
+
+
+def test_gpu_build_queries_equal_host_tree_and_oracle(gpu, oracle, ico4):
+    v, f = ico4
+    p = torch.from_numpy(S.uniform_points(30000, seed=5)).to(gpu)
+    g = cubvh_amd.cuBVH(v, f, build="gpu")
+    h = cubvh_amd.cuBVH(v, f)
+    for args in ((False,), (True,)):
+        a = g.unsigned_distance(p, return_uvw=args[0])
+        b = h.unsigned_distance(p, return_uvw=args[0])
+        for x, y in zip(a, b):
+            assert (x is None and y is None) or torch.equal(x, y)
+    od, of, ou = oracle.udf_brute(v, f, _np(p)[:3000])
+    d, fi, u = g.unsigned_distance(p[:3000], return_uvw=True)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(u), ou)
+    for mode in ("watertight", "raystab"):
+        a = g.signed_distance(p[:8192], mode=mode)
+        b = h.signed_distance(p[:8192], mode=mode)
+        assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    o, dirs = (torch.from_numpy(x).to(gpu) for x in S.sphere_rays(20000, seed=6))
+    a = g.ray_trace(o, dirs)
+    b = h.ray_trace(o, dirs)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_gpu_build_level6_and_reuse(gpu):
+    """A larger mesh; the GPU-built tree's state_dict imports, and it joins a forest
+    (whose host arrays it recomputes)."""
+    v, f, _ = S.perturbed_sphere(6, seed=1)
+    g = cubvh_amd.cuBVH(v, f, build="gpu")
+    h = cubvh_amd.cuBVH(v, f)
+    p = torch.from_numpy(S.grid_points(48)).to(gpu)
+    dg, fg, _ = g.unsigned_distance(p)
+    dh, fh, _ = h.unsigned_distance(p)
+    assert torch.equal(dg, dh) and torch.equal(fg, fh)
+    r = cubvh_amd.cuBVH.from_state_dict(g.state_dict())
+    dr, fr, _ = r.unsigned_distance(p)
+    assert torch.equal(dr, dh) and torch.equal(fr, fh)
+    forest = cubvh_amd.cuBVHForest([g, h])
+    pts = torch.cat([p, p])
+    offs = torch.tensor([0, p.shape[0], 2 * p.shape[0]], dtype=torch.int64, device=gpu)
+    d2, f2, _ = forest.unsigned_distance(pts, offs)
+    assert torch.equal(d2[: p.shape[0]], dh) and torch.equal(d2[p.shape[0]:], dh)
+    assert torch.equal(f2[: p.shape[0]], fh) and torch.equal(f2[p.shape[0]:], fh)
+
+
+def test_gpu_build_checks_input(gpu):
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    v = torch.rand(40, 3, device=gpu)
+    f = torch.randint(0, 40, (30, 3), dtype=torch.int32, device=gpu)
+    h = ctypes.c_void_p()
+    s = torch.cuda.current_stream(gpu).cuda_stream
+    f_bad = f.clone()
+    f_bad[17, 2] = 40
+    assert lib.cubvh_bvh_create_gpu(gpu.index, v.data_ptr(), 40, f_bad.data_ptr(), 30, 0, ctypes.byref(h), s) == -1
+    assert b"outside" in lib.cubvh_last_error()
+    v_bad = v.clone()
+    v_bad[int(f[4, 1]), 1] = float("inf")  # a vertex that a face uses (unused vertices are not read)
+    assert lib.cubvh_bvh_create_gpu(gpu.index, v_bad.data_ptr(), 40, f.data_ptr(), 30, 0, ctypes.byref(h), s) == -1
+    assert b"finite" in lib.cubvh_last_error()
+    # a mesh of at most leaf_size faces is one leaf (built on the host after a copy back)
+    assert lib.cubvh_bvh_create_gpu(gpu.index, v.data_ptr(), 40, f.data_ptr(), 5, 0, ctypes.byref(h), s) == 0
+    info = (ctypes.c_int64 * 5)()
+    assert lib.cubvh_bvh_info(h, info) == 0 and info[0] == 1 and info[1] == 5
+    lib.cubvh_bvh_destroy(h)
+    assert lib.cubvh_bvh_create_gpu(gpu.index, v.data_ptr(), 40, f.data_ptr(), 30, 0, ctypes.byref(h), s) == 0
+    lib.cubvh_bvh_destroy(h)
