@@ -22,6 +22,7 @@ SIGNATURES = {
         c_int,
         [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int32, POINTER(c_void_p), c_void_p],
     ),
+    "cubvh_bvh_export_reference": (c_int, [c_void_p, c_void_p, POINTER(c_int64), c_void_p]),
     "cubvh_bvh_destroy": (None, [c_void_p]),
     "cubvh_bvh_info": (c_int, [c_void_p, c_void_p]),
     "cubvh_bvh_export": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -98,10 +98,22 @@ class cuBVH:
         return {"n_nodes": arr[0], "n_faces": arr[1], "depth": arr[2], "stack_need": arr[3], "leaf_size": arr[4]}
 
     # ------------------------------------------------------- serialization
-    def state_dict(self):
+    def state_dict(self, layout: str = "own"):
         """dict of str -> CPU int32 tensor: 'nodes' [n_nodes, 32], 'triangles' [M, 12]
         (cubvh/api.py:26-28; bvh.cuh:84-94).  The triangle rows use the reference's
-        48-byte record (a, b, c fp32, 4 zero bytes, int64 face id)."""
+        48-byte record (a, b, c fp32, 4 zero bytes, int64 face id).  layout="reference"
+        writes the reference's own node layout (nodes [n, 9]), which the reference
+        package loads with its from_state_dict."""
+        if layout == "reference":
+            n9 = ctypes.c_int64(0)
+            _lib.check(self._lib.cubvh_bvh_export_reference(self._h, None, ctypes.byref(n9), None))
+            nodes = torch.empty(n9.value, 9, dtype=torch.int32)
+            tris = torch.empty(self.info()["n_faces"], 12, dtype=torch.int32)
+            _lib.check(self._lib.cubvh_bvh_export_reference(self._h, nodes.data_ptr(), ctypes.byref(n9),
+                                                            tris.data_ptr()))
+            return {"nodes": nodes, "triangles": tris}
+        if layout != "own":
+            raise ValueError(f"layout must be 'own' or 'reference', not {layout!r}")
         info = self.info()
         nodes = torch.empty(info["n_nodes"], 32, dtype=torch.int32)
         tris = torch.empty(info["n_faces"], 12, dtype=torch.int32)

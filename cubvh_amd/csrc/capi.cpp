@@ -510,6 +510,91 @@ int cubvh_bvh_export(const cubvh_bvh* bvh, int32_t* nodes, int32_t* triangles) {
     return CUBVH_OK;
 }
 
+// The reference's node (include/gpu/bvh.cuh:14-21): box (min, max), left, right,
+// escape.  Leaf: left = -begin - 1, right = -end - 1; internal: 4 children at
+// [left, right); escape links in pre-order (src/bvh.cu:595-614).
+struct RefNodeOut {
+    float mn[3], mx[3];
+    int32_t left, right, escape;
+};
+static_assert(sizeof(RefNodeOut) == 36, "reference node is 9 x 4 bytes");
+
+int cubvh_bvh_export_reference(const cubvh_bvh* bvh, int32_t* nodes9, int64_t* n_nodes9, int32_t* triangles) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh || !n_nodes9) return fail(CUBVH_ERR_ARG, "null argument");
+    const int64_t need = 1 + 4 * bvh->n_nodes;  // the root, then 4 children per wide node
+    if (!nodes9) {
+        *n_nodes9 = need;
+        return CUBVH_OK;
+    }
+    if (*n_nodes9 < need || !triangles) return fail(CUBVH_ERR_ARG, "node buffer too small or triangles null");
+    int rc = const_cast<cubvh_bvh*>(bvh)->ensure_host_mirror();  // a cache: logically const
+    if (rc != CUBVH_OK) return rc;
+    const HostBvh& h = bvh->host;
+    std::vector<RefNodeOut> out(1);
+    out.reserve((size_t)need);
+    float root_mn[3] = {INFINITY, INFINITY, INFINITY}, root_mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    out[0].left = out[0].right = out[0].escape = -1;
+    // breadth-first: our wide node w becomes the 4 consecutive reference nodes that
+    // follow its parent's entry; empty slots become empty leaves with an empty box
+    std::vector<std::pair<int32_t, int64_t>> todo = {{0, 0}};  // (wide node, its reference node)
+    for (size_t t = 0; t < todo.size(); ++t) {
+        const WideNode& w = h.nodes[(size_t)todo[t].first];
+        const int64_t me = todo[t].second;
+        const int64_t base = (int64_t)out.size();
+        out.resize(out.size() + 4);
+        out[(size_t)me].left = (int32_t)base;
+        out[(size_t)me].right = (int32_t)(base + 4);
+        for (int q = 0; q < 4; ++q) {
+            RefNodeOut& r = out[(size_t)(base + q)];
+            const int32_t ref = w.child[q];
+            r.escape = -1;
+            if (ref == CHILD_EMPTY) {
+                for (int k = 0; k < 3; ++k) {
+                    r.mn[k] = INFINITY;
+                    r.mx[k] = -INFINITY;
+                }
+                r.left = r.right = -1;  // begin == end == 0: no triangles
+                continue;
+            }
+            r.mn[0] = w.lo_x[q]; r.mn[1] = w.lo_y[q]; r.mn[2] = w.lo_z[q];
+            r.mx[0] = w.hi_x[q]; r.mx[1] = w.hi_y[q]; r.mx[2] = w.hi_z[q];
+            if (ref < 0) {
+                const int32_t first = (int32_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
+                const int32_t count = (ref & 7) + 1;
+                r.left = -first - 1;
+                r.right = -(first + count) - 1;
+            } else {
+                todo.push_back({ref, base + q});
+            }
+            if (me == 0)
+                for (int k = 0; k < 3; ++k) {
+                    root_mn[k] = std::min(root_mn[k], r.mn[k]);
+                    root_mx[k] = std::max(root_mx[k], r.mx[k]);
+                }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        out[0].mn[k] = root_mn[k];
+        out[0].mx[k] = root_mx[k];
+    }
+    // escape links: pre-order, child order 0..3 (next sibling, else the parent's escape)
+    std::vector<std::pair<int64_t, int32_t>> st = {{0, -1}};
+    while (!st.empty()) {
+        auto [ni, esc] = st.back();
+        st.pop_back();
+        RefNodeOut& nd = out[(size_t)ni];
+        nd.escape = esc;
+        if (nd.left < 0) continue;
+        for (int c = nd.right - 1; c >= nd.left; --c) st.push_back({c, c + 1 < nd.right ? c + 1 : esc});
+    }
+    std::memcpy(nodes9, out.data(), out.size() * sizeof(RefNodeOut));
+    *n_nodes9 = (int64_t)out.size();
+    for (int64_t i = 0; i < h.n_faces; ++i) export_record(&h.tris[12 * i], triangles + 12 * i);
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
 int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* triangles, int64_t n_faces,
                      cubvh_bvh** out) {
     CUBVH_GUARD_BEGIN

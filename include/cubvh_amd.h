@@ -86,6 +86,14 @@ int cubvh_bvh_export(const cubvh_bvh* bvh, int32_t* nodes, int32_t* triangles);
 int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* triangles, int64_t n_faces,
                      cubvh_bvh** out);
 
+/* Exports in the reference's state_dict layout (the inverse of its from_state_dict,
+ * src/api_gpu.cu:47-68; nodes bvh.cuh:14-21, escape links src/bvh.cu:595-614):
+ * nodes9 int32 [*n_nodes9, 9], triangles int32 [n_faces, 12] (leaf order).  Every
+ * wide node becomes 4 consecutive reference nodes (empty slots: empty leaves with an
+ * empty box), so the reference's traversals walk this tree.  With nodes9 NULL, only
+ * writes the node count needed to *n_nodes9 (1 + 4 * n_nodes). */
+int cubvh_bvh_export_reference(const cubvh_bvh* bvh, int32_t* nodes9, int64_t* n_nodes9, int32_t* triangles);
+
 /* Imports a state_dict written by the reference: nodes int32 [n_nodes, 9]
  * (TriangleBvhNode, bvh.cuh:14-21) and triangles int32 [n_faces, 12] (Triangle,
  * triangle.cuh:174-175).  The triangles (with their original face ids) are rebuilt

@@ -557,7 +557,8 @@ struct Stack32 {
 };
 
 // Stackless escape-link traversal (src/bvh.cu:178-213), used on stack overflow.
-std::pair<int, float> ref_closest_stackless(const RefBvh& b, V3 p, float maxd2, Counts* c, int tie_mode) {
+std::pair<int, float> ref_closest_stackless(const RefBvh& b, V3 p, float maxd2, Counts* c, int tie_mode,
+                                            bool* miss = nullptr) {
     float best = maxd2;
     int bi = -1;
     int idx = 0;
@@ -582,6 +583,7 @@ std::pair<int, float> ref_closest_stackless(const RefBvh& b, V3 p, float maxd2, 
             idx = nd.left_idx;
         }
     }
+    if (miss) *miss = (bi == -1);
     if (bi == -1) { bi = 0; best = 0.0f; }
     return {bi, std::sqrt(best)};
 }
@@ -1062,7 +1064,28 @@ void orc_refbvh_export(void* h, int32_t* nodes9, int32_t* tris12) {
     }
 }
 
-// Queries through the reference BVH.  tie_mode 0 = reference rule, 1 = canonical.
+// A reference-layout tree from state_dict arrays (src/api_gpu.cu:47-68): nodes int32
+// [n, 9], triangles int32 [M, 12].  No validation beyond the sizes: test input only.
+void* orc_refbvh_import(const int32_t* nodes9, int64_t n_nodes, const int32_t* tris12, int64_t n_tris) {
+    if (n_nodes <= 0 || n_tris <= 0) return nullptr;
+    RefBvh* b = new RefBvh;
+    b->nodes.resize((size_t)n_nodes);
+    std::memcpy(b->nodes.data(), nodes9, (size_t)n_nodes * sizeof(RefNode));
+    b->tris.resize((size_t)n_tris);
+    for (int64_t i = 0; i < n_tris; ++i) {
+        RefTriRecord r;
+        std::memcpy(&r, tris12 + 12 * i, sizeof r);
+        Tri& t = b->tris[(size_t)i];
+        t.a = v3(r.a[0], r.a[1], r.a[2]);
+        t.b = v3(r.b[0], r.b[1], r.b[2]);
+        t.c = v3(r.c[0], r.c[1], r.c[2]);
+        t.id = r.id;
+    }
+    return b;
+}
+
+// Queries through the reference BVH.  tie_mode 0 = reference rule, 1 = canonical;
+// plus 2 = walk with the escape links only.
 // counts (nullable): [box tests, triangle tests] summed over all queries.
 int orc_refbvh_udf(void* h, const float* pts, int64_t n, float* dist, int64_t* face, float* uvw, int tie_mode,
                    int64_t* counts, int nthreads) {
@@ -1074,9 +1097,12 @@ int orc_refbvh_udf(void* h, const float* pts, int64_t n, float* dist, int64_t* f
         V3 p = load3(pts, i);
         Counts c;
         bool miss = false;
-        auto r = ref_closest(b, p, MAX_DIST_SQ, counts ? &c : nullptr, tie_mode, &miss);
+        // tie_mode bit 1: walk with the escape links only (the reference's overflow path)
+        const int tm = tie_mode & 1;
+        auto r = (tie_mode & 2) ? ref_closest_stackless(b, p, MAX_DIST_SQ, counts ? &c : nullptr, tm, &miss)
+                                : ref_closest(b, p, MAX_DIST_SQ, counts ? &c : nullptr, tm, &miss);
         int ti = r.first;
-        if (miss && tie_mode == 1) {  // canonical: the miss path reports face 0
+        if (miss && tm == 1) {  // canonical: the miss path reports face 0
             for (size_t k = 0; k < b.tris.size(); ++k)
                 if (b.tris[k].id == 0) { ti = (int)k; break; }
         }
@@ -1099,7 +1125,8 @@ int orc_refbvh_ray(void* h, const float* ro, const float* rd, int64_t n, float* 
     for (int64_t i = 0; i < n; ++i) {
         V3 o = load3(ro, i), d = load3(rd, i);
         Counts c;
-        auto r = ref_ray(b, o, d, counts ? &c : nullptr, tie_mode);
+        auto r = (tie_mode & 2) ? ref_ray_stackless(b, o, d, counts ? &c : nullptr, tie_mode & 1)
+                                : ref_ray(b, o, d, counts ? &c : nullptr, tie_mode & 1);
         depth[i] = r.second;
         store3(pos, i, v3(std::fmaf(r.second, d.x, o.x), std::fmaf(r.second, d.y, o.y), std::fmaf(r.second, d.z, o.z)));
         face[i] = r.first >= 0 ? b.tris[r.first].id : -1;

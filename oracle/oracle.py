@@ -64,6 +64,7 @@ def load():
         "orc_ray_f64": (c_int, [vp, i64, vp, i64, vp, vp, i64, c_double, vp, vp, vp, c_int]),
         "orc_winding_f64": (c_int, [vp, i64, vp, i64, vp, i64, vp, c_int]),
         "orc_refbvh_build": (vp, [vp, i64, vp, i64]),
+        "orc_refbvh_import": (vp, [vp, i64, vp, i64]),
         "orc_refbvh_free": (None, [vp]),
         "orc_refbvh_num_nodes": (i64, [vp]),
         "orc_refbvh_export": (None, [vp, vp, vp]),
@@ -218,6 +219,19 @@ class RefBvh:
         if not self.h:
             raise ValueError("bad mesh")
 
+    @classmethod
+    def from_arrays(cls, nodes9, tris12):
+        """A tree in the reference's state_dict layout (nodes int32 [n, 9], triangles
+        int32 [M, 12]), walked by the restated reference traversals."""
+        self = cls.__new__(cls)
+        nodes9 = np.ascontiguousarray(nodes9, np.int32)
+        tris12 = np.ascontiguousarray(tris12, np.int32)
+        self.F = np.zeros((len(tris12), 3), np.int32)  # only its length is used
+        self.h = load().orc_refbvh_import(_p(nodes9), len(nodes9), _p(tris12), len(tris12))
+        if not self.h:
+            raise ValueError("empty tree")
+        return self
+
     def __del__(self):
         if getattr(self, "h", None):
             load().orc_refbvh_free(self.h)
@@ -235,6 +249,7 @@ class RefBvh:
         return nodes, tris
 
     def udf(self, pts, return_uvw=True, tie_mode=0, nthreads=0):
+        """tie_mode 0: the reference's rule; 1: canonical ties; +2: escape links only."""
         pts = _f32(pts).reshape(-1, 3)
         n = len(pts)
         d = np.empty(n, np.float32)

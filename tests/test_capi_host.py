@@ -254,3 +254,34 @@ def test_ordered_entry_points_validate_arguments(ico4):
     assert lib.cubvh_morton_order(0, 1 << 31, 1, 1, None) == -1
     assert b"at most" in lib.cubvh_last_error()
     assert lib.cubvh_morton_order(0, 0, None, None, None) == 0
+
+
+def test_export_reference_layout_walked_by_reference_traversal(oracle):
+    """state_dict(layout="reference") is a valid tree for the reference's own walks:
+    the oracle's restatement of closest_triangle / ray_intersect (stack walk and the
+    escape-link walk) over the exported arrays gives the brute-force answers."""
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    v, f, _ = S.perturbed_sphere(4, seed=7)
+    for leaf in (0, 3):
+        b = cubvh_amd.cuBVH(v, f, leaf_size=leaf)
+        sd = b.state_dict(layout="reference")
+        nodes, tris = sd["nodes"].numpy(), sd["triangles"].numpy()
+        assert nodes.shape == (1 + 4 * b.info()["n_nodes"], 9) and tris.shape == (len(f), 12)
+        R = oracle.RefBvh.from_arrays(nodes, tris)
+        p = S.uniform_points(3000, seed=leaf)
+        bd, bf, bu = oracle.udf_brute(v, f, p)
+        for mode in (1, 3):  # canonical ties; stack walk, then escape links only
+            d, fi, u, _ = R.udf(p, tie_mode=mode)
+            np.testing.assert_array_equal(d, bd)
+            np.testing.assert_array_equal(fi, bf)
+        ro, rd = S.sphere_rays(3000, seed=leaf)
+        bp, bfr, bt = oracle.ray_brute(v, f, ro, rd)
+        for mode in (1, 3):
+            pos, fr, t, _ = R.ray(ro, rd, tie_mode=mode)
+            np.testing.assert_array_equal(t, bt)
+            np.testing.assert_array_equal(fr, bfr)
+        # and it imports back (the reference layout importer rebuilds this layout)
+        b2 = cubvh_amd.cuBVH.from_state_dict(sd)
+        np.testing.assert_array_equal(b2._vertices.reshape(-1, 3, 3), v[f])
