@@ -475,10 +475,10 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     b->n_nodes = res.n_nodes;
     DeviceTree& dt = b->dev[device];
     e = dt.reserve(res.n_nodes, n_faces);
+    if (e == hipSuccess) e = gpu_build_arrays(n_faces, w, dt.view, hs);
     if (e == hipSuccess)
         e = hipMemcpyAsync(const_cast<WideNode*>(dt.view.nodes), w.wide, (size_t)res.n_nodes * sizeof(WideNode),
                            hipMemcpyDeviceToDevice, hs);
-    if (e == hipSuccess) e = gpu_build_arrays(n_faces, w, dt.view, hs);
     if (e == hipSuccess) e = gpu_build_face0_slot(w, h.face0_slot, hs);
     if (e != hipSuccess) {
         delete b;

@@ -11,17 +11,18 @@
 //                 parallelism in the construction of BVHs, octrees and k-d trees",
 //                 HPG 2012), equal codes ordered by position; one thread per
 //                 internal node;
-//   4. boxes    : bottom-up from the leaves; of the two threads reaching a node the
-//                 second computes its box (acquire-release counter per node);
-//   5. collapse : top-down, one launch per level: a wide node takes its binary
-//                 node's two children and opens the largest-area child until it has
-//                 4; subtrees of <= leaf_size faces become leaves;
-//   6. arrays   : leaf-ordered triangles, face / leaf / subtree bounding disks and
-//                 the per-face constants of the distance formula, with the layouts
-//                 and rules of the host build (bvh.hpp).
-// A subtree of the radix tree covers a contiguous run of sorted faces, so leaves
-// and subtree disks are ranges of the leaf order.  Child boxes are widened by the
-// host build's margin, so query results are the same whichever builder made the tree.
+//   4. collapse : top-down, one launch per level: a wide node takes its binary
+//                 node's two children and opens the child with the largest Morton
+//                 cell (shortest common prefix of its codes) until it has 4;
+//                 subtrees of <= leaf_size faces become leaves;
+//   5. arrays   : leaf-ordered triangles; then one pass over each child's run of
+//                 faces gives its box (written into its wide node) and its bounding
+//                 disk; face disks and the per-face constants of the distance
+//                 formula -- the layouts and rules of the host build (bvh.hpp).
+// A subtree of the radix tree covers a contiguous run of sorted faces, so child boxes
+// and disks are reductions over ranges of the leaf order, and no bottom-up pass (with
+// its cross-workgroup synchronization) is needed.  Child boxes are widened by the host
+// build's margin, so query results are the same whichever builder made the tree.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
@@ -49,11 +50,14 @@ __global__ void init_counters(int32_t* c) {
     else if (threadIdx.x < 16) c[threadIdx.x] = 0;                     // the rest
 }
 
-// Gathers face f's vertices into tri9[9f..], checks them, and folds its centroid and
-// |coordinates| into the counters.
+// Gathers face f's vertices into tri9[9f..] and checks them.  Each block writes its
+// partial centroid bounds, largest |coordinate| and error bits to part[8 * block]
+// (ordered ints; no atomics on shared counters); fold_partials combines them.
+constexpr int kGatherBlocks = 2048;
+
 __global__ __launch_bounds__(256) void gather_faces(int64_t n, const float* __restrict__ V, int64_t nv,
                                                     const int32_t* __restrict__ F, float* __restrict__ tri9,
-                                                    int32_t* c) {
+                                                    int32_t* __restrict__ part) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     float amax = 0.0f;
     int32_t err = 0;
@@ -79,14 +83,48 @@ __global__ __launch_bounds__(256) void gather_faces(int64_t n, const float* __re
             hi[a] = fmaxf(hi[a], cen);
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        if (lo[a] <= hi[a]) {
-            atomicMin(&c[a], ordered(lo[a]));
-            atomicMax(&c[3 + a], ordered(hi[a]));
+    int32_t v[8] = {ordered(lo[0]), ordered(lo[1]), ordered(lo[2]), ordered(hi[0]), ordered(hi[1]), ordered(hi[2]),
+                    __float_as_int(amax), err};
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 8; ++k) {
+            const int32_t x = __shfl_xor(v[k], o);
+            v[k] = k < 3 ? min(v[k], x) : k < 7 ? max(v[k], x) : (v[k] | x);
         }
+    __shared__ int32_t sh[4][8];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 8; ++k) sh[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        int32_t r = sh[0][k];
+        for (int w = 1; w < 4; ++w) r = k < 3 ? min(r, sh[w][k]) : k < 7 ? max(r, sh[w][k]) : (r | sh[w][k]);
+        part[8 * blockIdx.x + k] = r;
     }
-    if (amax > 0.0f) atomicMax(&c[6], __float_as_int(amax));  // non-negative: int order = float order
-    if (err) atomicOr(&c[7], err);
+}
+
+// One block: folds the gather partials into counters [0..7].
+__global__ __launch_bounds__(256) void fold_partials(const int32_t* __restrict__ part, int32_t nparts, int32_t* c) {
+    __shared__ int32_t sh[256];
+    for (int k = 0; k < 8; ++k) {
+        int32_t r = k < 3 ? 0x7fffffff : k < 7 ? (int32_t)0x80000000 : 0;
+        if (k == 6) r = 0;
+        for (int i = threadIdx.x; i < nparts; i += 256) {
+            const int32_t x = part[8 * i + k];
+            r = k < 3 ? min(r, x) : k < 7 ? max(r, x) : (r | x);
+        }
+        sh[threadIdx.x] = r;
+        __syncthreads();
+        for (int step = 128; step > 0; step >>= 1) {
+            if ((int)threadIdx.x < step) {
+                const int32_t x = sh[threadIdx.x + step];
+                sh[threadIdx.x] = k < 3 ? min(sh[threadIdx.x], x) : k < 7 ? max(sh[threadIdx.x], x) : (sh[threadIdx.x] | x);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) c[k] = sh[0];
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ uint64_t spread21(uint64_t x) {
@@ -153,6 +191,7 @@ __global__ __launch_bounds__(256) void radix_tree(int64_t n, const uint64_t* __r
     BinNode& b = bn[i];
     b.first = (int32_t)lo;
     b.last = (int32_t)hi;
+    b.prefix = dnode;
     b.left = lo == g ? ~(int32_t)g : (int32_t)g;
     b.right = hi == g + 1 ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
     if (b.left < 0) leaf_parent[g] = (int32_t)i;
@@ -160,55 +199,6 @@ __global__ __launch_bounds__(256) void radix_tree(int64_t n, const uint64_t* __r
     if (b.right < 0) leaf_parent[g + 1] = (int32_t)i;
     else bn[g + 1].parent = (int32_t)i;
     if (i == 0) b.parent = -1;
-}
-
-struct Box {
-    float lo[3], hi[3];
-};
-
-__device__ __forceinline__ Box face_box(const float* __restrict__ tri9, int64_t f) {
-    Box r;
-    for (int a = 0; a < 3; ++a) {
-        const float x = tri9[9 * f + a], y = tri9[9 * f + 3 + a], z = tri9[9 * f + 6 + a];
-        r.lo[a] = fminf(x, fminf(y, z));
-        r.hi[a] = fmaxf(x, fmaxf(y, z));
-    }
-    return r;
-}
-
-__device__ __forceinline__ Box node_box(const BinNode& b) {
-    Box r;
-    for (int a = 0; a < 3; ++a) {
-        r.lo[a] = b.lo[a];
-        r.hi[a] = b.hi[a];
-    }
-    return r;
-}
-
-__device__ __forceinline__ Box child_box(const BinNode* bn, const uint32_t* __restrict__ face_of,
-                                         const float* __restrict__ tri9, int32_t ref) {
-    return ref < 0 ? face_box(tri9, face_of[~ref]) : node_box(bn[ref]);
-}
-
-// One thread per leaf walks up; the second arrival at a node computes its box.
-__global__ __launch_bounds__(256) void boxes_up(int64_t n, const uint32_t* __restrict__ face_of,
-                                                const float* __restrict__ tri9, BinNode* bn,
-                                                const int32_t* __restrict__ leaf_parent, int32_t* arrivals) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    int32_t p = leaf_parent[j];
-    while (p >= 0) {
-        const int32_t before =
-            __hip_atomic_fetch_add(&arrivals[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (before == 0) return;  // the sibling subtree is not done yet: it continues
-        const Box a = child_box(bn, face_of, tri9, __hip_atomic_load(&bn[p].left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const Box b = child_box(bn, face_of, tri9, __hip_atomic_load(&bn[p].right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        for (int k = 0; k < 3; ++k) {
-            bn[p].lo[k] = fminf(a.lo[k], b.lo[k]);
-            bn[p].hi[k] = fmaxf(a.hi[k], b.hi[k]);
-        }
-        p = bn[p].parent;
-    }
 }
 
 // Outward rounding by a relative and an absolute margin (the host build's rule).
@@ -219,46 +209,85 @@ __device__ __forceinline__ float pad_up(float v, double abs_pad) {
     return __double2float_ru((double)v + fabs((double)v) * 1.0e-6 + abs_pad);
 }
 
-__device__ __forceinline__ float box_area(const Box& b) {
-    const float d0 = b.hi[0] - b.lo[0], d1 = b.hi[1] - b.lo[1], d2 = b.hi[2] - b.lo[2];
-    return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
+// Exclusive prefix sum of x over the lanes of the wave, and the wave's total.
+__device__ __forceinline__ int32_t wave_scan(int32_t x, int32_t& total) {
+    const int lane = (int)(threadIdx.x & 63);
+    int32_t v = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
+    total = __shfl(v, 63);
+    return v - x;
 }
 
-// One wide node per thread: entry (binary node, wide node, depth).
+// One wide node per thread: entry (binary node, wide node, depth).  Writes the
+// node's child references; the child boxes come later from the range passes.  The
+// output slots (wide node ids, next-level queue, leaf and subtree job lists) are
+// taken with one atomic per wave and counter.
 __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, int32_t n_in, int4* __restrict__ out,
-                                                      const BinNode* __restrict__ bn,
-                                                      const uint32_t* __restrict__ face_of,
-                                                      const float* __restrict__ tri9, int32_t leaf_size,
-                                                      WideNode* __restrict__ wn, int2* __restrict__ leaves,
+                                                      const BinNode* __restrict__ bn, int32_t leaf_size,
+                                                      WideNode* __restrict__ wn, int4* __restrict__ leaves,
                                                       int4* __restrict__ jobs, int32_t* c) {
     const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= n_in) return;
-    const int4 ent = in[e];
-    const BinNode& root = bn[ent.x];
-    int32_t kids[4] = {root.left, root.right, 0, 0};
-    int nk = 2;
+    const bool valid = e < n_in;
+    int4 ent = make_int4(0, 0, 0, 0);
+    int32_t kids[4] = {0, 0, 0, 0};
+    int nk = 0;
     auto size_of = [&](int32_t r) -> int32_t { return r < 0 ? 1 : bn[r].last - bn[r].first + 1; };
-    while (nk < 4) {
-        int best = -1;
-        float ba = -1.0f;
-        for (int q = 0; q < nk; ++q) {
-            const int32_t r = kids[q];
-            if (r >= 0 && size_of(r) > leaf_size) {
-                const float a = box_area(node_box(bn[r]));
-                if (a > ba) {
-                    ba = a;
+    if (valid) {
+        ent = in[e];
+        const BinNode& root = bn[ent.x];
+        kids[0] = root.left;
+        kids[1] = root.right;
+        nk = 2;
+        while (nk < 4) {
+            int best = -1, best_prefix = 1 << 30, best_size = 0;
+            for (int q = 0; q < nk; ++q) {
+                const int32_t r = kids[q];
+                if (r < 0) continue;
+                const int32_t sz = size_of(r);
+                if (sz <= leaf_size) continue;
+                const int32_t pf = bn[r].prefix;
+                if (pf < best_prefix || (pf == best_prefix && sz > best_size)) {
                     best = q;
+                    best_prefix = pf;
+                    best_size = sz;
                 }
             }
+            if (best < 0) break;
+            const int32_t o = kids[best];
+            for (int q = nk; q > best + 1; --q) kids[q] = kids[q - 1];
+            kids[best] = bn[o].left;
+            kids[best + 1] = bn[o].right;
+            ++nk;
         }
-        if (best < 0) break;
-        const int32_t o = kids[best];
-        for (int q = nk; q > best + 1; --q) kids[q] = kids[q - 1];
-        kids[best] = bn[o].left;
-        kids[best + 1] = bn[o].right;
-        ++nk;
     }
-    const double abs_pad = (double)(fmaxf(__int_as_float(c[6]), 1e-30f) * 4.0e-6f);
+    int32_t n_leaf = 0, n_int = 0;
+    for (int q = 0; q < nk; ++q) {
+        if (size_of(kids[q]) <= leaf_size) ++n_leaf;
+        else ++n_int;
+    }
+    int32_t tot_leaf, tot_int;
+    const int32_t off_leaf = wave_scan(n_leaf, tot_leaf);
+    const int32_t off_int = wave_scan(n_int, tot_int);
+    int32_t b_leaf = 0, b_wide = 0, b_queue = 0, b_job = 0;
+    if ((threadIdx.x & 63) == 0) {
+        if (tot_leaf) b_leaf = atomicAdd(&c[10], tot_leaf);
+        if (tot_int) {
+            b_wide = atomicAdd(&c[8], tot_int);
+            b_queue = atomicAdd(&c[9], tot_int);
+            b_job = atomicAdd(&c[11], tot_int);
+        }
+    }
+    b_leaf = __shfl(b_leaf, 0) + off_leaf;
+    b_wide = __shfl(b_wide, 0) + off_int;
+    b_queue = __shfl(b_queue, 0) + off_int;
+    b_job = __shfl(b_job, 0) + off_int;
+    int32_t depth = valid ? ent.z : 0;
+    for (int o = 32; o > 0; o >>= 1) depth = max(depth, __shfl_xor(depth, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&c[12], depth);
+    if (!valid) return;
     WideNode w;
     for (int q = 0; q < 4; ++q) {
         w.lo_x[q] = w.lo_y[q] = w.lo_z[q] = INFINITY;
@@ -269,27 +298,35 @@ __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ i
     w.pad[0] = w.pad[1] = w.pad[2] = 0;
     for (int q = 0; q < nk; ++q) {
         const int32_t r = kids[q];
-        const Box b = child_box(bn, face_of, tri9, r);
-        w.lo_x[q] = pad_down(b.lo[0], abs_pad);
-        w.lo_y[q] = pad_down(b.lo[1], abs_pad);
-        w.lo_z[q] = pad_down(b.lo[2], abs_pad);
-        w.hi_x[q] = pad_up(b.hi[0], abs_pad);
-        w.hi_y[q] = pad_up(b.hi[1], abs_pad);
-        w.hi_z[q] = pad_up(b.hi[2], abs_pad);
         const int32_t first = r < 0 ? ~r : bn[r].first;
         const int32_t cnt = size_of(r);
         if (cnt <= leaf_size) {
             w.child[q] = (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(cnt - 1));  // leaf reference
-            leaves[atomicAdd(&c[10], 1)] = make_int2(first, cnt);
+            leaves[b_leaf++] = make_int4(first, cnt, ent.y, q);
         } else {
-            const int32_t id = atomicAdd(&c[8], 1);
+            const int32_t id = b_wide++;
             w.child[q] = id;
-            out[atomicAdd(&c[9], 1)] = make_int4(r, id, ent.z + 1, 0);
-            jobs[atomicAdd(&c[11], 1)] = make_int4(ent.y, q, r, 0);
+            out[b_queue++] = make_int4(r, id, ent.z + 1, 0);
+            jobs[b_job++] = make_int4(ent.y, q, first, cnt);
         }
     }
     wn[ent.y] = w;
-    atomicMax(&c[12], ent.z);
+}
+
+// Writes child slot q of wide node w: the box of its faces widened by the margin.
+__device__ __forceinline__ void set_child_box(WideNode* __restrict__ wn, int32_t w, int32_t q, const double* lo,
+                                              const double* hi, double abs_pad) {
+    WideNode& n = wn[w];
+    n.lo_x[q] = pad_down((float)lo[0], abs_pad);
+    n.lo_y[q] = pad_down((float)lo[1], abs_pad);
+    n.lo_z[q] = pad_down((float)lo[2], abs_pad);
+    n.hi_x[q] = pad_up((float)hi[0], abs_pad);
+    n.hi_y[q] = pad_up((float)hi[1], abs_pad);
+    n.hi_z[q] = pad_up((float)hi[2], abs_pad);
+}
+
+__device__ __forceinline__ double margin_of(const int32_t* c) {
+    return (double)(fmaxf(__int_as_float(c[6]), 1e-30f) * 4.0e-6f);
 }
 
 // Leaf-ordered triangles (bvh.hpp) and the slot of original face 0.
@@ -427,15 +464,18 @@ __global__ __launch_bounds__(256) void face_records(int64_t n, const float4* __r
     o[8] = make_float4(g3.x, g3.y, g3.z, 0.0f);
 }
 
-// One thread per leaf (first, count): the leaf's disk at lbnd[2 * first].
-__global__ __launch_bounds__(256) void leaf_records(const int2* __restrict__ leaves, const int32_t* c,
-                                                    const float4* __restrict__ tris, float4* __restrict__ lbnd) {
+// One thread per leaf child (first, count, wide node, slot): its box into the wide
+// node and its disk at lbnd[2 * first].
+__global__ __launch_bounds__(256) void leaf_records(const int4* __restrict__ leaves, const int32_t* c,
+                                                    const float4* __restrict__ tris, float4* __restrict__ lbnd,
+                                                    WideNode* __restrict__ wn) {
     const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= c[10]) return;
-    const int2 lf = leaves[e];
+    const int4 lf = leaves[e];
     DiskAcc acc;
     acc_init(acc);
     for (int j = 0; j < lf.y; ++j) acc_face(acc, tris, lf.x + j);
+    set_child_box(wn, lf.z, lf.w, acc.lo, acc.hi, margin_of(c));
     double cen[3];
     for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (acc.lo[k] + acc.hi[k]);
     const DiskFrame fr = frame_of(cen, acc.n);
@@ -449,11 +489,11 @@ __global__ __launch_bounds__(256) void leaf_records(const int2* __restrict__ lea
 }
 
 // Structural check of the collapsed tree: every sorted face lies in exactly one leaf.
-__global__ __launch_bounds__(256) void cover_leaves(const int2* __restrict__ leaves, const int32_t* c,
+__global__ __launch_bounds__(256) void cover_leaves(const int4* __restrict__ leaves, const int32_t* c,
                                                     int32_t* cover) {
     const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= c[10]) return;
-    const int2 lf = leaves[e];
+    const int4 lf = leaves[e];
     for (int j = 0; j < lf.y; ++j) atomicAdd(&cover[lf.x + j], 1);
 }
 __global__ __launch_bounds__(256) void check_cover(int64_t n, const int32_t* __restrict__ cover, int32_t* c) {
@@ -463,15 +503,17 @@ __global__ __launch_bounds__(256) void check_cover(int64_t n, const int32_t* __r
 
 constexpr int kJobThreads = 256;
 
-__device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, bool is_max) {
-    // sh: kJobThreads * nv doubles
+enum { kSum = 0, kMin = 1, kMax = 2 };
+
+// Block-wide reduction of nv values per thread with one operation (sh: kJobThreads * 3).
+__device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, int op) {
     for (int k = 0; k < nv; ++k) sh[k * kJobThreads + threadIdx.x] = v[k];
     __syncthreads();
     for (int step = kJobThreads / 2; step > 0; step >>= 1) {
         if ((int)threadIdx.x < step)
             for (int k = 0; k < nv; ++k) {
                 const double a = sh[k * kJobThreads + threadIdx.x], b = sh[k * kJobThreads + threadIdx.x + step];
-                sh[k * kJobThreads + threadIdx.x] = is_max ? fmax(a, b) : a + b;
+                sh[k * kJobThreads + threadIdx.x] = op == kSum ? a + b : op == kMin ? fmin(a, b) : fmax(a, b);
             }
         __syncthreads();
     }
@@ -479,35 +521,73 @@ __device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, bool
     __syncthreads();
 }
 
-// One block per internal child (wide node, slot, binary node): the disk of every
-// face of the binary node's range, at cbnd[8 * wide + 2 * slot].
+// Internal children with more faces than this get a workgroup; the rest a wave.
+constexpr int32_t kWaveJobMax = 4096;
+
+__device__ __forceinline__ double wave_reduce(double v, int op) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double x = __shfl_xor(v, o);
+        v = op == kSum ? v + x : op == kMin ? fmin(v, x) : fmax(v, x);
+    }
+    return v;
+}
+
+// The box of an internal child's run of faces into its wide node and its disk at
+// cbnd[8 * wide + 2 * slot], job = (wide node, slot, first, count).  WAVE: one wave
+// per job of at most kWaveJobMax faces (shuffle reductions); else one workgroup per
+// larger job.
+template <bool WAVE>
 __global__ __launch_bounds__(kJobThreads) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
-                                                               const BinNode* __restrict__ bn,
                                                                const float4* __restrict__ tris,
-                                                               float4* __restrict__ cbnd) {
+                                                               float4* __restrict__ cbnd, WideNode* __restrict__ wn) {
     __shared__ double sh[3 * kJobThreads];
-    for (int32_t e = blockIdx.x; e < c[11]; e += gridDim.x) {
-        const int4 job = jobs[e];
-        const BinNode& b = bn[job.z];
-        double nsum[3] = {0.0, 0.0, 0.0};
-        for (int64_t s = b.first + (int64_t)threadIdx.x; s <= b.last; s += kJobThreads) {
-            DiskAcc acc;
-            acc_init(acc);
-            acc_face(acc, tris, s);
-            for (int k = 0; k < 3; ++k) nsum[k] += acc.n[k];
+    const double abs_pad = margin_of(c);
+    const int lane = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int width = WAVE ? 64 : kJobThreads;
+    const int32_t per_block = WAVE ? kJobThreads / 64 : 1;
+    const int32_t sub = WAVE ? (int32_t)(threadIdx.x >> 6) : 0;
+    for (int32_t e0 = blockIdx.x * per_block; e0 < c[11]; e0 += gridDim.x * per_block) {
+        const int32_t e = e0 + sub;
+        int4 job = make_int4(0, 0, 0, 0);
+        if (e < c[11]) job = jobs[e];
+        // each kernel takes only its own kind of job (uniform per wave / per block)
+        const bool mine = e < c[11] && (WAVE ? job.w <= kWaveJobMax : job.w > kWaveJobMax);
+        if (WAVE && !mine) continue;
+        if (!WAVE && !mine) continue;  // uniform over the block: one job per block
+        const int64_t first = job.z, end = (int64_t)job.z + job.w;
+        DiskAcc acc;
+        acc_init(acc);
+        for (int64_t s = first + lane; s < end; s += width) acc_face(acc, tris, s);
+        if (WAVE) {
+            for (int k = 0; k < 3; ++k) {
+                acc.lo[k] = wave_reduce(acc.lo[k], kMin);
+                acc.hi[k] = wave_reduce(acc.hi[k], kMax);
+                acc.n[k] = wave_reduce(acc.n[k], kSum);
+            }
+        } else {
+            block_reduce(sh, acc.lo, 3, kMin);
+            block_reduce(sh, acc.hi, 3, kMax);
+            block_reduce(sh, acc.n, 3, kSum);
         }
-        block_reduce(sh, nsum, 3, false);
         double cen[3];
-        for (int k = 0; k < 3; ++k) cen[k] = 0.5 * ((double)b.lo[k] + (double)b.hi[k]);
-        const DiskFrame fr = frame_of(cen, nsum);
+        for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (acc.lo[k] + acc.hi[k]);
+        const DiskFrame fr = frame_of(cen, acc.n);
         double ext[2] = {0.0, 0.0};
-        for (int64_t s = b.first + (int64_t)threadIdx.x; s <= b.last; s += kJobThreads) {
+        for (int64_t s = first + lane; s < end; s += width) {
             const double2 x = face_extent(fr, tris, s);
             ext[0] = fmax(ext[0], x.x);
             ext[1] = fmax(ext[1], x.y);
         }
-        block_reduce(sh, ext, 2, true);
-        if (threadIdx.x == 0) store_disk(fr, ext[0], ext[1], cbnd + 8 * (int64_t)job.x + 2 * job.y);
+        if (WAVE) {
+            ext[0] = wave_reduce(ext[0], kMax);
+            ext[1] = wave_reduce(ext[1], kMax);
+        } else {
+            block_reduce(sh, ext, 2, kMax);
+        }
+        if (lane == 0) {
+            store_disk(fr, ext[0], ext[1], cbnd + 8 * (int64_t)job.x + 2 * job.y);
+            set_child_box(wn, job.x, job.y, acc.lo, acc.hi, abs_pad);
+        }
     }
 }
 
@@ -519,10 +599,9 @@ size_t gpu_build_workspace_bytes(int64_t n) {
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63);
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const int64_t ni = n > 1 ? n - 1 : 1;
-    return al(64 * sizeof(int32_t)) + al(9 * n * sizeof(float)) + 2 * al(n * sizeof(uint64_t)) +
+    return al((64 + 8 * gb::kGatherBlocks) * sizeof(int32_t)) + al(9 * n * sizeof(float)) + 2 * al(n * sizeof(uint64_t)) +
            2 * al(n * sizeof(uint32_t)) + al(sort_tmp) + al(ni * sizeof(BinNode)) + al(n * sizeof(int32_t)) +
-           al(ni * sizeof(int32_t)) + al(ni * sizeof(WideNode)) + 2 * al(ni * sizeof(int4)) +
-           al(n * sizeof(int2)) + al(ni * sizeof(int4));
+           al(ni * sizeof(WideNode)) + 2 * al(ni * sizeof(int4)) + al(n * sizeof(int4)) + al(ni * sizeof(int4));
 }
 
 GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
@@ -533,7 +612,7 @@ GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
     const int64_t ni = n > 1 ? n - 1 : 1;
     char* p = (char*)ws;
     GpuBuildWorkspace w;
-    w.counters = (int32_t*)p; p += al(64 * sizeof(int32_t));
+    w.counters = (int32_t*)p; p += al((64 + 8 * gb::kGatherBlocks) * sizeof(int32_t));
     w.tri9 = (float*)p; p += al(9 * n * sizeof(float));
     w.keys0 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
     w.keys1 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
@@ -542,11 +621,10 @@ GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
     w.sort_tmp = p; w.sort_tmp_bytes = sort_tmp; p += al(sort_tmp);
     w.bn = (BinNode*)p; p += al(ni * sizeof(BinNode));
     w.leaf_parent = (int32_t*)p; p += al(n * sizeof(int32_t));
-    w.arrivals = (int32_t*)p; p += al(ni * sizeof(int32_t));
     w.wide = (WideNode*)p; p += al(ni * sizeof(WideNode));
     w.queue0 = (int4*)p; p += al(ni * sizeof(int4));
     w.queue1 = (int4*)p; p += al(ni * sizeof(int4));
-    w.leaves = (int2*)p; p += al(n * sizeof(int2));
+    w.leaves = (int4*)p; p += al(n * sizeof(int4));
     w.jobs = (int4*)p; p += al(ni * sizeof(int4));
     w.n = n;
     return w;
@@ -560,8 +638,9 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s) {
     hipError_t e = hipSuccess;
     hipLaunchKernelGGL(gb::init_counters, dim3(1), dim3(64), 0, s, w.counters);
-    hipLaunchKernelGGL(gb::gather_faces, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, n, V, nv, F,
-                       w.tri9, w.counters);
+    const unsigned gblocks = std::min<unsigned>(blocks_for(n), gb::kGatherBlocks);
+    hipLaunchKernelGGL(gb::gather_faces, dim3(gblocks), dim3(256), 0, s, n, V, nv, F, w.tri9, w.counters + 64);
+    hipLaunchKernelGGL(gb::fold_partials, dim3(1), dim3(256), 0, s, w.counters + 64, (int32_t)gblocks, w.counters);
     int32_t c[16];
     e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -575,9 +654,6 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (e != hipSuccess) return e;
     const int64_t ni = n - 1;
     hipLaunchKernelGGL(gb::radix_tree, dim3(blocks_for(ni)), dim3(256), 0, s, n, w.keys1, w.bn, w.leaf_parent);
-    hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(ni), 4096)), dim3(256), 0, s, w.arrivals, ni, 0);
-    hipLaunchKernelGGL(gb::boxes_up, dim3(blocks_for(n)), dim3(256), 0, s, n, w.face_of, w.tri9, w.bn, w.leaf_parent,
-                       w.arrivals);
     // collapse, one level per launch; the root entry is (binary root 0, wide node 0, depth 1)
     const int4 root = make_int4(0, 0, 1, 0);
     const int32_t one = 1;
@@ -593,7 +669,7 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
         e = hipMemcpyAsync(w.counters + 9, &zero, sizeof zero, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(gb::collapse_level, dim3(blocks_for(n_in)), dim3(256), 0, s, qin, n_in, qout, w.bn,
-                           w.face_of, w.tri9, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+                           leaf_size, w.wide, w.leaves, w.jobs, w.counters);
         e = hipMemcpyAsync(&n_in, w.counters + 9, sizeof n_in, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
@@ -629,9 +705,11 @@ hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArr
     hipLaunchKernelGGL(gb::face_records, dim3(blocks_for(n)), dim3(256), 0, s, n, out.tris,
                        const_cast<float4*>(out.tbnd), const_cast<float4*>(out.tpre));
     hipLaunchKernelGGL(gb::leaf_records, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, out.tris,
-                       const_cast<float4*>(out.lbnd));
-    hipLaunchKernelGGL(gb::subtree_records, dim3(2048), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters, w.bn,
-                       out.tris, const_cast<float4*>(out.cbnd));
+                       const_cast<float4*>(out.lbnd), w.wide);
+    hipLaunchKernelGGL(gb::subtree_records<true>, dim3(4096), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters,
+                       out.tris, const_cast<float4*>(out.cbnd), w.wide);
+    hipLaunchKernelGGL(gb::subtree_records<false>, dim3(1024), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters,
+                       out.tris, const_cast<float4*>(out.cbnd), w.wide);
     return hipGetLastError();
 }
 

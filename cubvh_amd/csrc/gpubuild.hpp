@@ -10,10 +10,10 @@
 namespace cubvh_amd {
 
 // Binary radix-tree node: children >= 0 are internal nodes, < 0 the leaf ~j (sorted
-// face j); [first, last] is the node's run of sorted faces; box of its vertices.
+// face j); [first, last] is the node's run of sorted faces; prefix is the length of
+// the common prefix of its codes (shorter = larger Morton cell).
 struct BinNode {
-    int32_t left, right, first, last, parent;
-    float lo[3], hi[3];
+    int32_t left, right, first, last, parent, prefix;
 };
 
 struct GpuBuildWorkspace {
@@ -30,12 +30,11 @@ struct GpuBuildWorkspace {
     size_t sort_tmp_bytes = 0;
     BinNode* bn = nullptr;        // [n-1]
     int32_t* leaf_parent = nullptr;
-    int32_t* arrivals = nullptr;
     WideNode* wide = nullptr;     // [n-1] wide nodes (the first `n_nodes` used)
     int4* queue0 = nullptr;
     int4* queue1 = nullptr;
-    int2* leaves = nullptr;       // (first, count) per leaf
-    int4* jobs = nullptr;         // (wide node, slot, binary node) per internal child
+    int4* leaves = nullptr;       // (first, count, wide node, slot) per leaf child
+    int4* jobs = nullptr;         // (wide node, slot, first, count) per internal child
     int64_t n = 0;
 };
 
@@ -48,13 +47,15 @@ struct GpuBuildResult {
 size_t gpu_build_workspace_bytes(int64_t n);
 GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n);
 
-// Steps 1-5 (n >= 2 faces): wide nodes in w.wide, leaf order in w.face_of.  Returns
+// Steps 1-4 (n >= 2 faces): wide nodes (without boxes) in w.wide, leaf order in
+// w.face_of.  Returns
 // with res.input_errors set (and nothing built) when an index or coordinate is bad.
 // Synchronizes the stream once per tree level.
 hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s);
 
-// Step 6: triangles and derived arrays into `out` (nodes are copied by the caller).
+// Step 5: triangles and derived arrays into `out`, child boxes into w.wide (the caller
+// copies w.wide to the tree's nodes afterwards).
 hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArrays& out, hipStream_t s);
 
 // Leaf-order slot of original face 0 (after gpu_build_arrays).  Synchronizes.
