@@ -143,6 +143,25 @@ __device__ __forceinline__ float tri_ray(f3 a, f3 b, f3 c, f3 ro, f3 rd) {
     return t;
 }
 
+// Both rays o + t rd and o - t rd (t >= 0) against triangle abc in one evaluation:
+// bit 0 set when tri_ray(a, b, c, ro, rd) < tmax, bit 1 when tri_ray(a, b, c, ro, -rd)
+// < tmax.  Negating rd negates q, den, the reciprocal and t exactly and leaves u and v
+// unchanged, so these are the two calls' results bit for bit.
+__device__ __forceinline__ int tri_line(f3 a, f3 b, f3 c, f3 ro, f3 rd, float tmax) {
+    f3 v1v0 = sub3(b, a), v2v0 = sub3(c, a), rov0 = sub3(ro, a);
+    f3 n = cross3(v1v0, v2v0);
+    f3 q = cross3(rov0, rd);
+    float den = dot3(rd, n);
+    if (den == 0.0f) return 0;
+    float d = 1.0f / den;
+    float u = d * -dot3(q, v2v0);
+    float v = d * dot3(q, v1v0);
+    float t = d * -dot3(n, rov0);
+    if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && (u + v) <= 1.0f)) return 0;
+    const float tn = -t;
+    return ((t >= 0.0f && t < tmax) ? 1 : 0) | ((tn >= 0.0f && tn < tmax) ? 2 : 0);
+}
+
 __device__ __forceinline__ f3 tri_normal(f3 a, f3 b, f3 c) {
     f3 n = cross3(sub3(b, a), sub3(c, a));
     float z = dot3(n, n);

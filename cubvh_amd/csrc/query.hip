@@ -290,40 +290,69 @@ __device__ int32_t closest_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N
     return bslot;
 }
 
-// true if some face is hit with t < MAX_DIST
+// Any hit along both rays o + t d and o - t d (t in [0, MAX_DIST)): bit 0 for +d, bit 1
+// for -d; the same answers as two separate any-hit walks, the reference's stab pair
+// (src/bvh.cu:414-424: a face hit with t < MAX_DIST along each).  One slab test gives
+// the line's interval [tn, tf] in a box: the +d ray meets it when tf >= 0 and
+// tn <= limit, the -d ray when tn <= 0 and tf >= -limit (the two rays' own slab tests,
+// negated exactly).  Children are walked nearest-first by the entry distance of the
+// directions still open; a stack entry carries those directions in the two low bits of
+// its key and is dropped at pop when both are answered.
 template <int BLOCK, int LDS_N>
-__device__ bool any_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
-    f3 inv = ray_inverse(d);
+__device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
+    const f3 inv = ray_inverse(d);
     const float limit = MAX_DIST * BOUND_SLACK;
+    int found = 0;
     int32_t ref = 0;
+    int dirs = 3;
     st.sp = 0;
     for (;;) {
-        if (ref >= 0) {
-            WideNode nd = load_node(T.nodes, ref);
-            float k[4];
-            int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
-            box_ray_4(nd, o, inv, limit, k);
-            sort4(k, r);
-            if (k[0] <= limit) {
-                if (k[3] <= limit) st.push(r[3], k[3]);
-                if (k[2] <= limit) st.push(r[2], k[2]);
-                if (k[1] <= limit) st.push(r[1], k[1]);
-                ref = r[0];
-                continue;
-            }
-        } else {
-            int32_t first, count;
-            leaf_range(ref, first, count);
-            for (int j = 0; j < count; ++j) {
-                f3 a, b, c;
-                int32_t id;
-                load_tri(T.tris, first + j, a, b, c, id);
-                if (tri_ray(a, b, c, o, d) < MAX_DIST) return true;
+        if ((dirs & ~found) != 0) {
+            if (ref >= 0) {
+                WideNode nd = load_node(T.nodes, ref);
+                float k[4];
+                int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
+                const int open = 3 & ~found;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float tx0 = (nd.lo_x[j] - o.x) * inv.x, tx1 = (nd.hi_x[j] - o.x) * inv.x;
+                    float ty0 = (nd.lo_y[j] - o.y) * inv.y, ty1 = (nd.hi_y[j] - o.y) * inv.y;
+                    float tz0 = (nd.lo_z[j] - o.z) * inv.z, tz1 = (nd.hi_z[j] - o.z) * inv.z;
+                    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+                    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+                    const bool box = nd.child[j] != CHILD_EMPTY && tn <= tf;
+                    const int m = box ? (((tf >= 0.0f && tn <= limit) ? 1 : 0) | ((tn <= 0.0f && tf >= -limit) ? 2 : 0)) & open
+                                      : 0;
+                    float key = INFINITY;
+                    if (m & 1) key = fmaxf(tn, 0.0f);
+                    if (m & 2) key = fminf(key, fmaxf(-tf, 0.0f));
+                    k[j] = m ? __uint_as_float((__float_as_uint(key) & ~3u) | (uint32_t)m) : INFINITY;
+                }
+                sort4(k, r);
+                if (k[0] != INFINITY) {
+                    if (k[3] != INFINITY) st.push(r[3], k[3]);
+                    if (k[2] != INFINITY) st.push(r[2], k[2]);
+                    if (k[1] != INFINITY) st.push(r[1], k[1]);
+                    ref = r[0];
+                    dirs = (int)(__float_as_uint(k[0]) & 3u);
+                    continue;
+                }
+            } else {
+                int32_t first, count;
+                leaf_range(ref, first, count);
+                for (int j = 0; j < count && (dirs & ~found) != 0; ++j) {
+                    f3 a, b, c;
+                    int32_t id;
+                    load_tri(T.tris, first + j, a, b, c, id);
+                    found |= tri_line(a, b, c, o, d, MAX_DIST) & dirs;
+                }
+                if (found == 3) return 3;
             }
         }
-        if (st.sp == 0) return false;
+        if (st.sp == 0) return found;
         --st.sp;
         ref = st.ref_at(st.sp);
+        dirs = (int)(__float_as_uint(st.key_at(st.sp)) & 3u);
     }
 }
 
@@ -663,7 +692,7 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
         bool inside = true;
         for (uint32_t k = 0; k < 32; ++k) {
             f3 dir = fibonacci_dir32(k, ox, oy);
-            if (!any_hit(T, p, neg3(dir), st) || !any_hit(T, p, dir, st)) {
+            if (any_hit_line(T, p, dir, st) != 3) {
                 inside = false;
                 break;
             }
@@ -703,7 +732,7 @@ __global__ __launch_bounds__(BLOCK) void stab_sign_kernel(int64_t n, const float
     float oy = rng.next_float();
     for (uint32_t k = 0; k < 32; ++k) {
         f3 dir = fibonacci_dir32(k, ox, oy);
-        if (!any_hit(T, p, neg3(dir), st) || !any_hit(T, p, dir, st)) return;  // a miss: outside
+        if (any_hit_line(T, p, dir, st) != 3) return;  // a miss along +dir or -dir: outside
     }
     distances[q] = -distances[q];
 }

@@ -243,3 +243,37 @@ def test_ordered_distance_equals_plain_calls(gpu, oracle):
     keep = torch.ones(n, dtype=torch.bool, device=gpu)
     keep[victim] = False
     assert torch.equal(dist[keep], d0[keep]) and torch.equal(face[keep], f0[keep])
+
+
+def test_duplicate_faces_tie_to_lowest_index(gpu, oracle, ico4):
+    """Every face appears twice (and some three times) with higher indices: every
+    distance, ray depth and sign ties exactly, and the answer must be the lowest
+    index -- the original face -- in every query form and on both builders."""
+    v, f = ico4
+    dup = np.concatenate([f, f[::-1], f[:500]]).astype(np.int32)
+    p = S.uniform_points(20000, seed=12)
+    P = torch.from_numpy(p).to(gpu)
+    ref = cubvh_amd.cuBVH(v, f)
+    d0, f0, u0 = ref.unsigned_distance(P, return_uvw=True)
+    o, dirs = S.sphere_rays(20000, seed=13)
+    O, Dd = torch.from_numpy(o).to(gpu), torch.from_numpy(dirs).to(gpu)
+    r0 = ref.ray_trace(O, Dd)
+    s0 = ref.signed_distance(P, mode="watertight")
+    for build in ("host", "gpu"):
+        b = cubvh_amd.cuBVH(v, dup, build=build)
+        d, fi, u = b.unsigned_distance(P, return_uvw=True)
+        assert torch.equal(d, d0) and torch.equal(fi, f0) and torch.equal(u, u0)
+        r = b.ray_trace(O, Dd)
+        for x, y in zip(r, r0):
+            assert torch.equal(x, y)
+        # the sign averages the normals of the faces at the closest point, duplicates
+        # included (as the reference does): compare with the oracle on this mesh
+        s = b.signed_distance(P, mode="watertight")
+        assert torch.equal(s[0].abs(), s0[0].abs()) and torch.equal(s[1], s0[1])
+        osd, osf, _ = oracle.sdf_brute(v, dup, p[:2000], 0, return_uvw=False)
+        np.testing.assert_array_equal(_np(s[0])[:2000], osd)
+        np.testing.assert_array_equal(_np(s[1])[:2000], osf)
+    od, of, _ = oracle.udf_brute(v, dup, p[:2000])
+    np.testing.assert_array_equal(_np(d0)[:2000], od)
+    np.testing.assert_array_equal(_np(f0)[:2000], of)
+    assert int(f0.max()) < len(f)
