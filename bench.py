@@ -165,8 +165,10 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
     import cubvh_amd
     from cubvh_amd import synthetic as S
 
+    import ctypes
     from concurrent.futures import ThreadPoolExecutor
 
+    lib = cubvh_amd._lib.load()
     specs = S.forest(n_meshes, seed=0)
     t0 = time.perf_counter()
     rng = np.random.default_rng(1)
@@ -174,12 +176,16 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
     for lvl in sorted({sp["level"] for sp in specs}):
         S.icosphere_unit(lvl)  # shared topology, built once per level
 
-    def make(sp):
-        v, f, _ = S.perturbed_sphere(sp["level"], seed=sp["seed"], scale=sp["scale"])
-        return cubvh_amd.cuBVH(v, f)  # the host build runs in the library without the GIL
+    def mesh(sp):
+        return S.perturbed_sphere(sp["level"], seed=sp["seed"], scale=sp["scale"])[:2]
 
     with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        bvhs = list(ex.map(make, specs))
+        meshes = list(ex.map(mesh, specs))
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        # the host build runs in the library without the GIL
+        bvhs = list(ex.map(lambda m: cubvh_amd.cuBVH(*m), meshes))
     build_s = time.perf_counter() - t0
     counts = np.array([sp["n_queries"] for sp in specs])
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
@@ -209,9 +215,37 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
         ok &= bool(torch.equal(graph.dist[offsets[m]:offsets[m + 1]], d_ref))
         ok &= bool(torch.equal(graph.face[offsets[m]:offsets[m + 1]], f_ref))
     best = min(t_forest, t_graph)
+    del graph
+    # the same meshes built on the GPU (already resident there) and assembled on the device
+    mesh_dev = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f in meshes]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gbvhs = []
+    for v, f in mesh_dev:
+        h = ctypes.c_void_p()
+        cubvh_amd._lib.check(lib.cubvh_bvh_create_gpu(dev.index, v.data_ptr(), v.shape[0], f.data_ptr(), f.shape[0],
+                                                      0, ctypes.byref(h), torch.cuda.current_stream(dev).cuda_stream))
+        gbvhs.append(h)
+    gforest = ctypes.c_void_p()
+    arr = (ctypes.c_void_p * n_meshes)(*[h.value for h in gbvhs])
+    cubvh_amd._lib.check(lib.cubvh_forest_create(arr, n_meshes, ctypes.byref(gforest)))
+    gpu_build_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    cubvh_amd._lib.check(lib.cubvh_forest_reserve(gforest, dev.index, n, tuning[0], stream))
+    tun = (ctypes.c_int32 * 3)(*tuning)
+
+    def gpu_forest():
+        cubvh_amd._lib.check(lib.cubvh_forest_unsigned_distance(gforest, dev.index, n, P.data_ptr(), off.data_ptr(),
+                                                                dist.data_ptr(), face.data_ptr(), None, tun, stream))
+
+    _, t_gforest = _time_kernel(gpu_forest, steps, warmup)
+    lib.cubvh_forest_destroy(gforest)
+    for h in gbvhs:
+        lib.cubvh_bvh_destroy(h)
     return {"workload": f"{n_meshes} perturbed icospheres, {int(forest.info()['n_faces'])} faces, "
                         f"{n} uniform points in [-1,1]^3 (4,096..262,144 per mesh)",
-            "queries": int(n), "faces": int(forest.info()["n_faces"]), "host_build_s": build_s,
+            "queries": int(n), "faces": int(forest.info()["n_faces"]), "mesh_generation_s": gen_s,
+            "host_build_s": build_s, "gpu_build_s": gpu_build_s, "gpu_built_forest_Mq_s": n / t_gforest / 1e6,
             "plain_ms": t_plain * 1e3, "forest_ms": t_forest * 1e3, "graph_ms": t_graph * 1e3,
             "plain_Mq_s": n / t_plain / 1e6, "batched_Mq_s": n / best / 1e6, "speedup": t_plain / best,
             "tuning": list(tuning), "equal_to_per_mesh": ok}

@@ -220,7 +220,10 @@ struct cubvh_forest {
     std::vector<int32_t> mesh_info;  // 4 per mesh: root, face-0 slot, n_faces, 0
     int32_t n_meshes = 0;
     int64_t n_faces = 0;
+    int64_t n_nodes = 0;
     int32_t stack_need = 0;
+    // >= 0: assembled on that device from trees built there; no host arrays
+    int32_t resident_device = -1;
     std::mutex mu;
     struct DevCopy {
         DeviceTree tree;
@@ -247,6 +250,9 @@ struct cubvh_forest {
     int upload(int device, hipStream_t stream) {
         DevCopy& c = dev[device];
         if (c.tree.block) return CUBVH_OK;
+        if (resident_device >= 0)
+            return fail(CUBVH_ERR_ARG, "this forest was assembled on device " + std::to_string(resident_device) +
+                                           " from GPU-built trees and is used there only");
         hipError_t e = hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device);
         if (e != hipSuccess) return fail_hip(e, "hipDeviceGetAttribute");
         const size_t mb = mesh_info.size() * sizeof(int32_t);
@@ -829,19 +835,89 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
     CUBVH_GUARD_END
 }
 
+namespace {
+
+// Forest of trees that all live on `device` only (GPU-built): their device arrays are
+// copied into one block there, node references rebased by a kernel.
+int forest_create_on_device(cubvh_bvh* const* bvhs, int32_t n_meshes, int32_t device, cubvh_forest** out) {
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    cubvh_forest* f = new cubvh_forest;
+    f->n_meshes = n_meshes;
+    f->resident_device = device;
+    f->mesh_info.assign(4 * (size_t)n_meshes, 0);
+    for (int32_t m = 0; m < n_meshes; ++m) {
+        const cubvh_bvh* b = bvhs[m];
+        if (f->n_faces + b->host.n_faces > MAX_LEAF_FIRST || f->n_nodes + b->n_nodes > 0x7fffffff) {
+            delete f;
+            return fail(CUBVH_ERR_ARG, "forest too large for the 28-bit leaf encoding");
+        }
+        f->mesh_info[4 * m + 0] = (int32_t)f->n_nodes;
+        f->mesh_info[4 * m + 1] = (int32_t)(f->n_faces + b->host.face0_slot);
+        f->mesh_info[4 * m + 2] = (int32_t)b->host.n_faces;
+        f->n_nodes += b->n_nodes;
+        f->n_faces += b->host.n_faces;
+        f->stack_need = std::max(f->stack_need, b->host.stack_need);
+    }
+    if (stack_capacity_for(f->stack_need) < 0) {
+        delete f;
+        return fail(CUBVH_ERR_STATE, "tree too deep for the traversal stack");
+    }
+    cubvh_forest::DevCopy& c = f->dev[device];
+    hipError_t e = hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = c.tree.reserve(f->n_nodes, f->n_faces);
+    const size_t mb = f->mesh_info.size() * sizeof(int32_t);
+    if (e == hipSuccess) e = hipMalloc(&c.mesh_info, mb);
+    if (e == hipSuccess) e = hipMemcpy(c.mesh_info, f->mesh_info.data(), mb, hipMemcpyHostToDevice);
+    const hipStream_t s = nullptr;
+    int64_t node_base = 0, tri_base = 0;
+    for (int32_t m = 0; m < n_meshes && e == hipSuccess; ++m) {
+        const cubvh_bvh* b = bvhs[m];
+        const TreeArrays& src = b->dev[device].view;
+        const TreeArrays& dst = c.tree.view;
+        const int64_t nf = b->host.n_faces, nn = b->n_nodes;
+        auto cp = [&](const void* to, const void* from, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpyAsync(const_cast<void*>(to), from, bytes, hipMemcpyDeviceToDevice, s);
+        };
+        cp(dst.nodes + node_base, src.nodes, (size_t)nn * sizeof(WideNode));
+        cp(dst.tris + 3 * tri_base, src.tris, (size_t)nf * 3 * sizeof(float4));
+        cp(dst.tbnd + 2 * tri_base, src.tbnd, (size_t)nf * 2 * sizeof(float4));
+        cp(dst.lbnd + 2 * tri_base, src.lbnd, (size_t)nf * 2 * sizeof(float4));
+        cp(dst.cbnd + 8 * node_base, src.cbnd, (size_t)nn * 8 * sizeof(float4));
+        cp(dst.tpre + 9 * tri_base, src.tpre, (size_t)nf * 9 * sizeof(float4));
+        if (e == hipSuccess)
+            e = launch_rebase_nodes(const_cast<WideNode*>(dst.nodes) + node_base, nn, (int32_t)node_base,
+                                    (int32_t)tri_base, s);
+        node_base += nn;
+        tri_base += nf;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        delete f;
+        return fail_hip(e, "forest assembly on the device");
+    }
+    *out = f;
+    return CUBVH_OK;
+}
+
+}  // namespace
+
 int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest** out) {
     CUBVH_GUARD_BEGIN
     if (!out) return fail(CUBVH_ERR_ARG, "out is null");
     *out = nullptr;
     if (!bvhs || n_meshes <= 0) return fail(CUBVH_ERR_ARG, "need at least one BVH");
+    for (int32_t m = 0; m < n_meshes; ++m)
+        if (!bvhs[m]) return fail(CUBVH_ERR_ARG, "null BVH in the list");
+    // every tree built on one device and not copied to the host: assemble there
+    const int32_t d0 = bvhs[0]->mirror_device;
+    bool on_device = d0 >= 0;
+    for (int32_t m = 1; m < n_meshes && on_device; ++m) on_device = bvhs[m]->mirror_device == d0;
+    if (on_device) return forest_create_on_device(bvhs, n_meshes, d0, out);
     cubvh_forest* f = new cubvh_forest;
     f->n_meshes = n_meshes;
     f->mesh_info.assign(4 * (size_t)n_meshes, 0);
     for (int32_t m = 0; m < n_meshes; ++m) {
-        if (!bvhs[m]) {
-            delete f;
-            return fail(CUBVH_ERR_ARG, "null BVH in the list");
-        }
         int rc = bvhs[m]->ensure_host_derived();
         if (rc != CUBVH_OK) {
             delete f;
@@ -878,6 +954,7 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
         f->n_faces += h.n_faces;
         f->stack_need = std::max(f->stack_need, h.stack_need);
     }
+    f->n_nodes = (int64_t)f->nodes.size();
     if (stack_capacity_for(f->stack_need) < 0) {
         delete f;
         return fail(CUBVH_ERR_STATE, "tree too deep for the traversal stack");
@@ -892,7 +969,7 @@ void cubvh_forest_destroy(cubvh_forest* forest) { delete forest; }
 int cubvh_forest_info(const cubvh_forest* forest, int64_t info[4]) {
     if (!forest || !info) return fail(CUBVH_ERR_ARG, "null argument");
     info[0] = forest->n_meshes;
-    info[1] = (int64_t)forest->nodes.size();
+    info[1] = forest->n_nodes;
     info[2] = forest->n_faces;
     info[3] = forest->stack_need;
     return CUBVH_OK;

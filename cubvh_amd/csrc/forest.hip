@@ -163,6 +163,35 @@ __global__ void reset_counter(uint32_t* c) {
 
 }  // namespace dev
 
+namespace dev {
+
+// Child references of n nodes of one mesh moved into the forest: internal nodes by
+// node_base, leaf ranges by tri_base.
+__global__ __launch_bounds__(256) void rebase_nodes(WideNode* nodes, int64_t n, int32_t node_base, int32_t tri_base) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    WideNode& w = nodes[i];
+    for (int k = 0; k < 4; ++k) {
+        const int32_t r = w.child[k];
+        if (r == CHILD_EMPTY) continue;
+        if (r >= 0) {
+            w.child[k] = r + node_base;
+        } else {
+            const uint32_t first = (((uint32_t)r & 0x7fffffffu) >> 3) + (uint32_t)tri_base;
+            w.child[k] = (int32_t)(0x80000000u | (first << 3) | ((uint32_t)r & 7u));
+        }
+    }
+}
+
+}  // namespace dev
+
+hipError_t launch_rebase_nodes(WideNode* nodes, int64_t n, int32_t node_base, int32_t tri_base, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::rebase_nodes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nodes, n, node_base,
+                       tri_base);
+    return hipGetLastError();
+}
+
 size_t forest_workspace_bytes(int64_t n, int32_t n_meshes, int32_t bits) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const int64_t npad = n + 64 * (int64_t)n_meshes;

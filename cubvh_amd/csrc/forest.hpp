@@ -47,4 +47,7 @@ struct ForestArgs {
 
 hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws, hipStream_t s);
 
+// Rebases the child references of n nodes copied into a forest (forest.hip).
+hipError_t launch_rebase_nodes(WideNode* nodes, int64_t n, int32_t node_base, int32_t tri_base, hipStream_t s);
+
 }  // namespace cubvh_amd

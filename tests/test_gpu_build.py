@@ -124,3 +124,24 @@ def test_gpu_build_checks_input(gpu):
     lib.cubvh_bvh_destroy(h)
     assert lib.cubvh_bvh_create_gpu(gpu.index, v.data_ptr(), 40, f.data_ptr(), 30, 0, ctypes.byref(h), s) == 0
     lib.cubvh_bvh_destroy(h)
+
+
+def test_forest_of_gpu_built_trees_assembled_on_device(gpu):
+    """Trees that live only on the device form a forest there (no host copy): results
+    equal the per-mesh calls, and the same forest from host-built trees."""
+    meshes = [S.perturbed_sphere(l, seed=s)[:2] for l, s in ((3, 1), (5, 2), (4, 3))]
+    g = [cubvh_amd.cuBVH(v, f, build="gpu") for v, f in meshes]
+    h = [cubvh_amd.cuBVH(v, f) for v, f in meshes]
+    fg = cubvh_amd.cuBVHForest(g)
+    fh = cubvh_amd.cuBVHForest(h)
+    assert fg.info()["n_faces"] == sum(len(f) for _, f in meshes)
+    assert fg.info()["n_nodes"] == sum(b.info()["n_nodes"] for b in g)
+    pts = [torch.from_numpy(S.uniform_points(n, seed=n)).to(gpu) for n in (3000, 9000, 100)]
+    P = torch.cat(pts)
+    offs = torch.tensor([0, 3000, 12000, 12100], dtype=torch.int64, device=gpu)
+    dg, fgi, ug = fg.unsigned_distance(P, offs, return_uvw=True)
+    dh, fhi, uh = fh.unsigned_distance(P, offs, return_uvw=True)
+    assert torch.equal(dg, dh) and torch.equal(fgi, fhi) and torch.equal(ug, uh)
+    for m, (b, p) in enumerate(zip(g, pts)):
+        d, fi, _ = b.unsigned_distance(p)
+        assert torch.equal(dg[offs[m]:offs[m + 1]], d) and torch.equal(fgi[offs[m]:offs[m + 1]], fi)
