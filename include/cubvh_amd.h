@@ -176,9 +176,12 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
  * ------------------------------------------------------------------------- */
 typedef struct cubvh_forest cubvh_forest;
 
-/* Concatenates the trees of n_meshes BVHs (host only; the BVHs stay owned by the
- * caller and may be destroyed afterwards).  Face ids reported by the forest are the
- * per-mesh original face indices. */
+/* Concatenates the trees of n_meshes BVHs (the BVHs stay owned by the caller and may
+ * be destroyed afterwards).  Face ids reported by the forest are the per-mesh
+ * original face indices.  Host-built trees are concatenated on the host and uploaded
+ * on first use.  When every tree was built by cubvh_bvh_create_gpu on one device and
+ * none has been copied to the host, the forest is assembled on that device (device
+ * copies, synchronous) and serves queries on that device only. */
 int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest** out);
 void cubvh_forest_destroy(cubvh_forest* forest);
 
