@@ -794,6 +794,7 @@ int cubvh_ray_trace(cubvh_bvh* bvh, int device, int64_t n, const float* rays_o, 
     q.out_v = positions;
     q.out_id = face_id;
     q.out_f = depth;
+    q.packet = n >= kMortonMin;  // packets from 2,048 rays on, in the caller's order
     hipError_t e = launch_ray(q);
     if (e != hipSuccess) return fail_hip(e, "ray kernel launch");
     return CUBVH_OK;

@@ -643,6 +643,82 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
     }
 }
 
+// Packet form of closest_hit: every lane traces its own ray; node and face records
+// are wave-uniform.  Children are ordered by the entry distance of their lowest
+// interested lane.  Results (t, slot, id) as closest_hit.
+__device__ __forceinline__ float slab_entry(float lx, float hx, float ly, float hy, float lz, float hz, f3 o, f3 inv) {
+    const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+    const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+    const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    return (tn <= tf && tf >= 0.0f) ? tn : INFINITY;
+}
+
+__device__ void closest_hit_packet(const Tree& T, f3 o, f3 d, bool active, float& best, int32_t& bslot,
+                                   int32_t& bid) {
+    const int lane = (int)(threadIdx.x & 63);
+    const f3 inv = ray_inverse(d);
+    best = MAX_DIST;
+    float bound = MAX_DIST * BOUND_SLACK;
+    bslot = -1;
+    bid = 0x7fffffff;
+    uint64_t mask = ballot(active);
+    if (mask == 0) return;
+    WaveStack st;
+    int32_t ref = 0;
+    for (;;) {
+        if (ref >= 0) {
+            UNode nd = load_unode(T.nodes, ref);
+            float k0 = nd.child.x != CHILD_EMPTY ? slab_entry(nd.lx.x, nd.hx.x, nd.ly.x, nd.hy.x, nd.lz.x, nd.hz.x, o, inv) : INFINITY;
+            float k1 = nd.child.y != CHILD_EMPTY ? slab_entry(nd.lx.y, nd.hx.y, nd.ly.y, nd.hy.y, nd.lz.y, nd.hz.y, o, inv) : INFINITY;
+            float k2 = nd.child.z != CHILD_EMPTY ? slab_entry(nd.lx.z, nd.hx.z, nd.ly.z, nd.hy.z, nd.lz.z, nd.hz.z, o, inv) : INFINITY;
+            float k3 = nd.child.w != CHILD_EMPTY ? slab_entry(nd.lx.w, nd.hx.w, nd.ly.w, nd.hy.w, nd.lz.w, nd.hz.w, o, inv) : INFINITY;
+            uint64_t m0 = ballot(k0 <= bound) & mask;
+            uint64_t m1 = ballot(k1 <= bound) & mask;
+            uint64_t m2 = ballot(k2 <= bound) & mask;
+            uint64_t m3 = ballot(k3 <= bound) & mask;
+            if ((m0 | m1 | m2 | m3) != 0) {
+                // order by entry distance clamped at 0 (a box around the origin enters at
+                // a negative t); non-negative floats compare as their bit patterns
+                uint32_t r0 = rep_key(fmaxf(k0, 0.0f), m0), r1 = rep_key(fmaxf(k1, 0.0f), m1);
+                uint32_t r2 = rep_key(fmaxf(k2, 0.0f), m2), r3 = rep_key(fmaxf(k3, 0.0f), m3);
+                int32_t c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+                ucswap(r0, c0, m0, r2, c2, m2);
+                ucswap(r1, c1, m1, r3, c3, m3);
+                ucswap(r0, c0, m0, r1, c1, m1);
+                ucswap(r2, c2, m2, r3, c3, m3);
+                ucswap(r1, c1, m1, r2, c2, m2);
+                if (m3) st.push(c3, m3);
+                if (m2) st.push(c2, m2);
+                if (m1) st.push(c1, m1);
+                ref = uniform(c0);
+                mask = m0;
+                continue;
+            }
+        } else {
+            int32_t first, count;
+            leaf_range(ref, first, count);
+            for (int j = 0; j < count; ++j) {
+                f3 a, b, c;
+                int32_t id;
+                load_utri(T.tris, first + j, a, b, c, id);
+                if ((mask >> lane) & 1) {
+                    const float t = tri_ray(a, b, c, o, d);
+                    if (t < best || (t == best && bslot >= 0 && id < bid)) {
+                        best = t;
+                        bslot = first + j;
+                        bid = id;
+                        bound = best * BOUND_SLACK + 1e-30f;
+                    }
+                }
+            }
+        }
+        if (!st.pop(ref, mask)) break;
+        ref = uniform(ref);
+    }
+}
+
 __device__ __forceinline__ f3 load_pt(const float* __restrict__ p, int64_t i) {
     return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
@@ -870,6 +946,62 @@ __global__ __launch_bounds__(BLOCK) void ray_kernel(int64_t n, const float* __re
     face_id[i] = fid;
 }
 
+// Rays in the caller's order, one wave per 64.  A wave whose rays are nearly parallel
+// and start close together (camera rays in pixel order: directions within ~5.7 degrees
+// of its first ray, origins within 1% of the scene's diagonal) walks as a packet;
+// any other wave walks lane by lane.  Both give closest_hit's results.
+constexpr int kRayLds = 8;
+__global__ __launch_bounds__(256) void ray_packet_kernel(int64_t n, const float* __restrict__ rays_o,
+                                                         const float* __restrict__ rays_d,
+                                                         float* __restrict__ positions, int64_t* __restrict__ face_id,
+                                                         float* __restrict__ depth, Tree T) {
+    __shared__ int32_t s_ref[kRayLds * 256];
+    __shared__ float s_key[kRayLds * 256];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    f3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(1.0f, 0.0f, 0.0f);
+    if (valid) {
+        o = load_pt(rays_o, i);
+        d = load_pt(rays_d, i);
+    }
+    // coherence of the wave, judged against its first ray
+    const f3 o0 = mk3(__shfl(o.x, 0), __shfl(o.y, 0), __shfl(o.z, 0));
+    const f3 d0 = mk3(__shfl(d.x, 0), __shfl(d.y, 0), __shfl(d.z, 0));
+    const UNode root = load_unode(T.nodes, 0);
+    float diag2 = 0.0f;
+    {
+        const float lx = fminf(fminf(root.lx.x, root.lx.y), fminf(root.lx.z, root.lx.w));
+        const float hx = fmaxf(fmaxf(root.hx.x, root.hx.y), fmaxf(root.hx.z, root.hx.w));
+        const float ly = fminf(fminf(root.ly.x, root.ly.y), fminf(root.ly.z, root.ly.w));
+        const float hy = fmaxf(fmaxf(root.hy.x, root.hy.y), fmaxf(root.hy.z, root.hy.w));
+        const float lz = fminf(fminf(root.lz.x, root.lz.y), fminf(root.lz.z, root.lz.w));
+        const float hz = fmaxf(fmaxf(root.hz.x, root.hz.y), fmaxf(root.hz.z, root.hz.w));
+        // empty slots hold +inf / -inf bounds and drop out of the min / max
+        diag2 = (hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz);
+    }
+    const f3 od = sub3(o, o0);
+    const float dd = dot3(d, d0), n2 = dot3(d, d) * dot3(d0, d0);
+    const bool near = !valid || (dd > 0.0f && dd * dd >= 0.99f * n2 && dot3(od, od) <= 1e-4f * diag2);
+    float t;
+    int32_t slot, bid;
+    if (ballot(near) == ~0ull) {
+        closest_hit_packet(T, o, d, valid, t, slot, bid);
+    } else {
+        if (!valid) return;
+        LaneStack<256, kRayLds> st;
+        st.ref = s_ref;
+        st.key = s_key;
+        st.tid = (int)threadIdx.x;
+        st.sp = 0;
+        slot = closest_hit(T, o, d, st, t);
+        bid = slot >= 0 ? __float_as_int(T.tris[3 * slot].w) : 0;
+    }
+    if (!valid) return;
+    depth[i] = t;
+    store_pt(positions, i, mk3(__builtin_fmaf(t, d.x, o.x), __builtin_fmaf(t, d.y, o.y), __builtin_fmaf(t, d.z, o.z)));
+    face_id[i] = slot >= 0 ? (int64_t)bid : -1;
+}
+
 }  // namespace dev
 
 namespace {
@@ -948,6 +1080,12 @@ hipError_t launch_ray(const QueryArgs& q) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
+    if (q.packet) {
+        const unsigned grid = (unsigned)((q.n + 255) / 256);
+        hipLaunchKernelGGL(dev::ray_packet_kernel, dim3(grid), dim3(256), 0, (hipStream_t)q.stream, q.n, q.in0, q.in1,
+                           q.out_v, q.out_id, q.out_f, T);
+        return hipGetLastError();
+    }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((dev::ray_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, (hipStream_t)q.stream, q.n,
                        q.in0, q.in1, q.out_v, q.out_id, q.out_f, T);
