@@ -514,6 +514,7 @@ struct ExactQueue {
     uint32_t lane[QCAP];
     unsigned long long key[64];  // per query lane: (squared distance bits, face id), min wins
     int32_t kslot[64];           // per query lane: slot of the face holding key
+    uint32_t lanes[64];          // the lanes that reach the current leaf, in lane order
 };
 
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
@@ -563,6 +564,10 @@ __device__ __forceinline__ void run_exact(const Tree& T, ExactQueue& q, int32_t&
     qn = base;
     wave_fence();
 }
+
+// A leaf wanted by more lanes than this is tested one face at a time (a round of 64
+// pairs per face would cost more than the step); fewer lanes are packed into rounds.
+constexpr int kStepLanes = 32;
 
 // Nearest face for every lane of the wave (lanes with active = false only ride
 // along).  Results in the lane's best / bslot / bid, as closest_face.  q is this
@@ -618,20 +623,58 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
             // lanes whose point may still reach a face of this leaf
             cfloat_ptr lp = (cfloat_ptr)(T.lbnd + 2 * first);
             const uint64_t lm = ballot(!disk_rejects(p, ld4(lp), ld4(lp + 4), bound)) & mask;
-            for (int j = 0; lm != 0 && j < count; ++j) {
-                cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
-                const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & lm;
-                if (cm == 0) continue;  // no lane can reach this face
-                if ((cm >> lane) & 1) {
-                    const uint32_t at = (uint32_t)qn + lanes_below(cm);
-                    q.slot[at] = (uint32_t)(first + j);
-                    q.lane[at] = (uint32_t)lane;
+            if (__builtin_popcountll(lm) > kStepLanes) {
+                // most lanes want the leaf: one step per face, face records uniform
+                for (int j = 0; j < count; ++j) {
+                    cfloat_ptr bp = (cfloat_ptr)(T.tbnd + 2 * (first + j));
+                    const uint64_t cm = ballot(!disk_rejects(p, ld4(bp), ld4(bp + 4), bound)) & lm;
+                    if (cm == 0) continue;  // no lane can reach this face
+                    if ((cm >> lane) & 1) {
+                        const uint32_t at = (uint32_t)qn + lanes_below(cm);
+                        q.slot[at] = (uint32_t)(first + j);
+                        q.lane[at] = (uint32_t)lane;
+                    }
+                    qn += __builtin_popcountll(cm);
+                    if (qn >= 64) {
+                        wave_fence();
+                        run_exact(T, q, qn, p, best, bound, bslot, bid);
+                    }
                 }
-                qn += __builtin_popcountll(cm);
-                if (qn >= 64) {
-                    wave_fence();
-                    run_exact(T, q, qn, p, best, bound, bslot, bid);
+            } else if (lm != 0) {
+                // The (lane, face) pairs of the leaf, 64 per round: pair i is face
+                // i / nl of the (i % nl)-th lane of lm.  Pairs whose face disk does
+                // not reject join the exact-test queue.
+                const int32_t nl = __builtin_popcountll(lm);
+                if ((lm >> lane) & 1) q.lanes[lanes_below(lm)] = (uint32_t)lane;
+                wave_fence();
+                const float inv_nl = 1.0f / (float)nl;
+                const int32_t total = nl * count;
+                for (int32_t base = 0; base < total; base += 64) {
+                    const int32_t i = base + lane;
+                    const bool has = i < total;
+                    // (i + 1/2) / nl is at least 1/(2 nl) away from an integer: exact floor
+                    const int32_t j = has ? (int32_t)(((float)i + 0.5f) * inv_nl) : 0;
+                    const uint32_t src = has ? q.lanes[i - j * nl] : 0u;
+                    const f3 ps = mk3(__shfl(p.x, (int)src), __shfl(p.y, (int)src), __shfl(p.z, (int)src));
+                    const float bs = __shfl(bound, (int)src);
+                    bool pass = false;
+                    if (has) {
+                        const float4* bp = T.tbnd + 2 * (first + j);
+                        pass = !disk_rejects(ps, bp[0], bp[1], bs);
+                    }
+                    const uint64_t pm = ballot(pass);
+                    if (pass) {
+                        const uint32_t at = (uint32_t)qn + lanes_below(pm);
+                        q.slot[at] = (uint32_t)(first + j);
+                        q.lane[at] = src;
+                    }
+                    qn += __builtin_popcountll(pm);
+                    if (qn >= 64) {
+                        wave_fence();
+                        run_exact(T, q, qn, p, best, bound, bslot, bid);
+                    }
                 }
+                wave_fence();
             }
         }
         if (!st.pop(ref, mask)) break;
