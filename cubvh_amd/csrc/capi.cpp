@@ -468,7 +468,7 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     if (e != hipSuccess) return fail_hip(e, "GPU tree build");
     if (res.input_errors & 1) return fail(CUBVH_ERR_ARG, "a face references a vertex outside [0, n_vertices)");
     if (res.input_errors & 2) return fail(CUBVH_ERR_ARG, "vertex coordinates must be finite");
-    if (stack_capacity_for(3 * res.depth + 1) < 0)
+    if (res.too_deep || stack_capacity_for(3 * res.depth + 1) < 0)
         return fail(CUBVH_ERR_STATE, "GPU-built tree too deep for the traversal stack (depth " +
                                          std::to_string(res.depth) + "); use the host build");
     if (!res.structure_ok) return fail(CUBVH_ERR_STATE, "GPU-built tree failed its structure check");

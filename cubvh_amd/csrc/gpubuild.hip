@@ -47,7 +47,7 @@ __global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
 __global__ void init_counters(int32_t* c) {
     if (threadIdx.x < 3) c[threadIdx.x] = 0x7fffffff;                  // centroid lo (ordered)
     else if (threadIdx.x < 6) c[threadIdx.x] = (int32_t)0x80000000;    // centroid hi (ordered)
-    else if (threadIdx.x < 16) c[threadIdx.x] = 0;                     // the rest
+    else if (threadIdx.x < 64) c[threadIdx.x] = 0;                     // the rest
 }
 
 // Gathers face f's vertices into tri9[9f..] and checks them.  Each block writes its
@@ -225,11 +225,11 @@ __device__ __forceinline__ int32_t wave_scan(int32_t x, int32_t& total) {
 // node's child references; the child boxes come later from the range passes.  The
 // output slots (wide node ids, next-level queue, leaf and subtree job lists) are
 // taken with one atomic per wave and counter.
-__global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, int32_t n_in, int4* __restrict__ out,
-                                                      const BinNode* __restrict__ bn, int32_t leaf_size,
-                                                      WideNode* __restrict__ wn, int4* __restrict__ leaves,
-                                                      int4* __restrict__ jobs, int32_t* c) {
-    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int32_t e, int32_t n_in,
+                                               int4* __restrict__ out, int32_t* n_out,
+                                               const BinNode* __restrict__ bn, int32_t leaf_size,
+                                               WideNode* __restrict__ wn, int4* __restrict__ leaves,
+                                               int4* __restrict__ jobs, int32_t* c) {
     const bool valid = e < n_in;
     int4 ent = make_int4(0, 0, 0, 0);
     int32_t kids[4] = {0, 0, 0, 0};
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ i
         if (tot_leaf) b_leaf = atomicAdd(&c[10], tot_leaf);
         if (tot_int) {
             b_wide = atomicAdd(&c[8], tot_int);
-            b_queue = atomicAdd(&c[9], tot_int);
+            b_queue = atomicAdd(n_out, tot_int);
             b_job = atomicAdd(&c[11], tot_int);
         }
     }
@@ -311,6 +311,16 @@ __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ i
         }
     }
     wn[ent.y] = w;
+}
+
+__global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, const int32_t* n_in_ptr,
+                                                      int4* __restrict__ out, int32_t* n_out,
+                                                      const BinNode* __restrict__ bn, int32_t leaf_size,
+                                                      WideNode* __restrict__ wn, int4* __restrict__ leaves,
+                                                      int4* __restrict__ jobs, int32_t* c) {
+    const int32_t n_in = n_in_ptr ? *n_in_ptr : 1;  // the first level holds the root alone
+    for (int32_t e0 = (int32_t)(blockIdx.x * blockDim.x); e0 < n_in; e0 += (int32_t)(gridDim.x * blockDim.x))
+        collapse_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, bn, leaf_size, wn, leaves, jobs, c);
 }
 
 // Writes child slot q of wide node w: the box of its faces widened by the margin.
@@ -632,6 +642,7 @@ GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
 
 namespace {
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+constexpr int kMaxDepth = 21;  // 3 * 21 + 1 = 64 stack entries
 }  // namespace
 
 hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
@@ -641,12 +652,6 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     const unsigned gblocks = std::min<unsigned>(blocks_for(n), gb::kGatherBlocks);
     hipLaunchKernelGGL(gb::gather_faces, dim3(gblocks), dim3(256), 0, s, n, V, nv, F, w.tri9, w.counters + 64);
     hipLaunchKernelGGL(gb::fold_partials, dim3(1), dim3(256), 0, s, w.counters + 64, (int32_t)gblocks, w.counters);
-    int32_t c[16];
-    e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    res.input_errors = c[7];
-    if (c[7]) return hipSuccess;
     hipLaunchKernelGGL(gb::morton_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.counters, w.keys0,
                        w.vals0);
     size_t tmp = w.sort_tmp_bytes;
@@ -654,35 +659,35 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (e != hipSuccess) return e;
     const int64_t ni = n - 1;
     hipLaunchKernelGGL(gb::radix_tree, dim3(blocks_for(ni)), dim3(256), 0, s, n, w.keys1, w.bn, w.leaf_parent);
-    // collapse, one level per launch; the root entry is (binary root 0, wide node 0, depth 1)
+    // collapse: one launch per level for the deepest tree a walk accepts (depth
+    // kMaxDepth, 3 * depth + 1 <= 64 stack entries), without host synchronization: level
+    // k reads its entry count from counters[16 + k - 1] (the root alone at level 0) and
+    // counts its outputs in counters[16 + k]; a tree still growing after the last
+    // level is reported as too deep.  The root entry is (binary root 0, wide node 0, 1).
     const int4 root = make_int4(0, 0, 1, 0);
     const int32_t one = 1;
     e = hipMemcpyAsync(w.queue0, &root, sizeof root, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(w.counters + 8, &one, sizeof one, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
-    int4* qin = w.queue0;
-    int4* qout = w.queue1;
-    int32_t n_in = 1;
-    for (int level = 0; n_in > 0; ++level) {
-        if (level > 64) return hipErrorInvalidValue;  // deeper than any stack accepts
-        const int32_t zero = 0;
-        e = hipMemcpyAsync(w.counters + 9, &zero, sizeof zero, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(gb::collapse_level, dim3(blocks_for(n_in)), dim3(256), 0, s, qin, n_in, qout, w.bn,
-                           leaf_size, w.wide, w.leaves, w.jobs, w.counters);
-        e = hipMemcpyAsync(&n_in, w.counters + 9, sizeof n_in, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return e;
-        std::swap(qin, qout);
+    const unsigned cblocks = std::min<unsigned>(blocks_for(n), 1024);
+    for (int level = 0; level < kMaxDepth; ++level) {
+        const int4* qin = (level & 1) ? w.queue1 : w.queue0;
+        int4* qout = (level & 1) ? w.queue0 : w.queue1;
+        hipLaunchKernelGGL(gb::collapse_level, dim3(cblocks), dim3(256), 0, s, qin,
+                           level == 0 ? (const int32_t*)nullptr : w.counters + 16 + level - 1, qout,
+                           w.counters + 16 + level, w.bn, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
     }
     // every sorted face in exactly one leaf (leaf_parent is free again: reused as counts)
     hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, w.leaf_parent, n,
                        0);
     hipLaunchKernelGGL(gb::cover_leaves, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, w.leaf_parent);
     hipLaunchKernelGGL(gb::check_cover, dim3(blocks_for(n)), dim3(256), 0, s, n, w.leaf_parent, w.counters);
+    int32_t c[64];
     e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    res.input_errors = c[7];
+    res.too_deep = c[16 + kMaxDepth - 1] != 0;
     res.structure_ok = c[14] == 0;
     res.n_nodes = c[8];
     res.n_leaves = c[10];

@@ -19,8 +19,9 @@ struct BinNode {
 struct GpuBuildWorkspace {
     int32_t* counters = nullptr;  // [0..5] centroid bounds (ordered ints), [6] max |coordinate| bits,
                                   // [7] input errors (1 bad index, 2 non-finite), [8] wide nodes,
-                                  // [9] next queue length, [10] leaves, [11] subtree jobs,
-                                  // [12] depth, [13] slot of face 0, [14] structure error
+                                  // [10] leaves, [11] subtree jobs, [12] depth, [13] slot of
+                                  // face 0, [14] structure error, [16 + k] entries made by
+                                  // collapse level k; then the gather partials
     float* tri9 = nullptr;        // [9n] gathered vertices, original face order
     uint64_t* keys0 = nullptr;
     uint64_t* keys1 = nullptr;    // sorted Morton codes
@@ -42,15 +43,16 @@ struct GpuBuildResult {
     int32_t input_errors = 0;
     int32_t n_nodes = 0, n_leaves = 0, n_jobs = 0, depth = 0;
     bool structure_ok = false;  // every face in exactly one leaf
+    bool too_deep = false;      // deeper than a walk's stack accepts
 };
 
 size_t gpu_build_workspace_bytes(int64_t n);
 GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n);
 
 // Steps 1-4 (n >= 2 faces): wide nodes (without boxes) in w.wide, leaf order in
-// w.face_of.  Returns
-// with res.input_errors set (and nothing built) when an index or coordinate is bad.
-// Synchronizes the stream once per tree level.
+// w.face_of.  res.input_errors is set when an index or coordinate is bad (the tree is
+// then meaningless, but every access stayed in bounds); res.too_deep when the tree is
+// deeper than a walk's stack accepts.  Synchronizes the stream once.
 hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s);
 
