@@ -511,23 +511,29 @@ __global__ __launch_bounds__(256) void check_cover(int64_t n, const int32_t* __r
     if (s < n && cover[s] != 1) atomicOr(&c[14], 1);
 }
 
-constexpr int kJobThreads = 256;
 
 enum { kSum = 0, kMin = 1, kMax = 2 };
 
-// Block-wide reduction of nv values per thread with one operation (sh: kJobThreads * 3).
+// Workgroup-wide reduction of nv values per thread with one operation: waves reduce
+// with shuffles, then wave 0 combines the per-wave results (sh: 3 * 16 doubles).
+__device__ __forceinline__ double op_of(double a, double b, int op) {
+    return op == kSum ? a + b : op == kMin ? fmin(a, b) : fmax(a, b);
+}
+template <int BLOCK>
 __device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, int op) {
-    for (int k = 0; k < nv; ++k) sh[k * kJobThreads + threadIdx.x] = v[k];
-    __syncthreads();
-    for (int step = kJobThreads / 2; step > 0; step >>= 1) {
-        if ((int)threadIdx.x < step)
-            for (int k = 0; k < nv; ++k) {
-                const double a = sh[k * kJobThreads + threadIdx.x], b = sh[k * kJobThreads + threadIdx.x + step];
-                sh[k * kJobThreads + threadIdx.x] = op == kSum ? a + b : op == kMin ? fmin(a, b) : fmax(a, b);
-            }
-        __syncthreads();
+    constexpr int NW = BLOCK / 64;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    for (int k = 0; k < nv; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x = op_of(x, __shfl_xor(x, o), op);
+        if (lane == 0) sh[k * NW + wave] = x;
     }
-    for (int k = 0; k < nv; ++k) v[k] = sh[k * kJobThreads];
+    __syncthreads();
+    for (int k = 0; k < nv; ++k) {
+        double x = sh[k * NW];
+        for (int w = 1; w < NW; ++w) x = op_of(x, sh[k * NW + w], op);
+        v[k] = x;
+    }
     __syncthreads();
 }
 
@@ -546,15 +552,15 @@ __device__ __forceinline__ double wave_reduce(double v, int op) {
 // cbnd[8 * wide + 2 * slot], job = (wide node, slot, first, count).  WAVE: one wave
 // per job of at most kWaveJobMax faces (shuffle reductions); else one workgroup per
 // larger job.
-template <bool WAVE>
-__global__ __launch_bounds__(kJobThreads) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
-                                                               const float4* __restrict__ tris,
-                                                               float4* __restrict__ cbnd, WideNode* __restrict__ wn) {
-    __shared__ double sh[3 * kJobThreads];
+template <bool WAVE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
+                                                         const float4* __restrict__ tris,
+                                                         float4* __restrict__ cbnd, WideNode* __restrict__ wn) {
+    __shared__ double sh[3 * (BLOCK / 64)];
     const double abs_pad = margin_of(c);
     const int lane = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-    const int width = WAVE ? 64 : kJobThreads;
-    const int32_t per_block = WAVE ? kJobThreads / 64 : 1;
+    const int width = WAVE ? 64 : BLOCK;
+    const int32_t per_block = WAVE ? BLOCK / 64 : 1;
     const int32_t sub = WAVE ? (int32_t)(threadIdx.x >> 6) : 0;
     for (int32_t e0 = blockIdx.x * per_block; e0 < c[11]; e0 += gridDim.x * per_block) {
         const int32_t e = e0 + sub;
@@ -567,7 +573,12 @@ __global__ __launch_bounds__(kJobThreads) void subtree_records(const int4* __res
         const int64_t first = job.z, end = (int64_t)job.z + job.w;
         DiskAcc acc;
         acc_init(acc);
-        for (int64_t s = first + lane; s < end; s += width) acc_face(acc, tris, s);
+        // four faces per thread in flight: the loads do not wait on each other
+        for (int64_t s = first + lane; s < end; s += 4 * width) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (s + u * width < end) acc_face(acc, tris, s + u * width);
+        }
         if (WAVE) {
             for (int k = 0; k < 3; ++k) {
                 acc.lo[k] = wave_reduce(acc.lo[k], kMin);
@@ -575,24 +586,28 @@ __global__ __launch_bounds__(kJobThreads) void subtree_records(const int4* __res
                 acc.n[k] = wave_reduce(acc.n[k], kSum);
             }
         } else {
-            block_reduce(sh, acc.lo, 3, kMin);
-            block_reduce(sh, acc.hi, 3, kMax);
-            block_reduce(sh, acc.n, 3, kSum);
+            block_reduce<BLOCK>(sh, acc.lo, 3, kMin);
+            block_reduce<BLOCK>(sh, acc.hi, 3, kMax);
+            block_reduce<BLOCK>(sh, acc.n, 3, kSum);
         }
         double cen[3];
         for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (acc.lo[k] + acc.hi[k]);
         const DiskFrame fr = frame_of(cen, acc.n);
         double ext[2] = {0.0, 0.0};
-        for (int64_t s = first + lane; s < end; s += width) {
-            const double2 x = face_extent(fr, tris, s);
-            ext[0] = fmax(ext[0], x.x);
-            ext[1] = fmax(ext[1], x.y);
+        for (int64_t s = first + lane; s < end; s += 4 * width) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (s + u * width < end) {
+                    const double2 x = face_extent(fr, tris, s + u * width);
+                    ext[0] = fmax(ext[0], x.x);
+                    ext[1] = fmax(ext[1], x.y);
+                }
         }
         if (WAVE) {
             ext[0] = wave_reduce(ext[0], kMax);
             ext[1] = wave_reduce(ext[1], kMax);
         } else {
-            block_reduce(sh, ext, 2, kMax);
+            block_reduce<BLOCK>(sh, ext, 2, kMax);
         }
         if (lane == 0) {
             store_disk(fr, ext[0], ext[1], cbnd + 8 * (int64_t)job.x + 2 * job.y);
@@ -711,10 +726,10 @@ hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArr
                        const_cast<float4*>(out.tbnd), const_cast<float4*>(out.tpre));
     hipLaunchKernelGGL(gb::leaf_records, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, out.tris,
                        const_cast<float4*>(out.lbnd), w.wide);
-    hipLaunchKernelGGL(gb::subtree_records<true>, dim3(4096), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters,
-                       out.tris, const_cast<float4*>(out.cbnd), w.wide);
-    hipLaunchKernelGGL(gb::subtree_records<false>, dim3(1024), dim3(gb::kJobThreads), 0, s, w.jobs, w.counters,
-                       out.tris, const_cast<float4*>(out.cbnd), w.wide);
+    hipLaunchKernelGGL((gb::subtree_records<true, 256>), dim3(4096), dim3(256), 0, s, w.jobs, w.counters, out.tris,
+                       const_cast<float4*>(out.cbnd), w.wide);
+    hipLaunchKernelGGL((gb::subtree_records<false, 1024>), dim3(512), dim3(1024), 0, s, w.jobs, w.counters, out.tris,
+                       const_cast<float4*>(out.cbnd), w.wide);
     return hipGetLastError();
 }
 
