@@ -7,6 +7,7 @@ points (cubvh_morton_order), then walk them (cubvh_distance_ordered) -- the two 
 of cubvh_unsigned_distance, whose own rate is reported as config.dropin_call_Mq_s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline] [--headline-only]
+                 [--hashtable-only]
 
 Prints one JSON line.  `value` = M queries/s over the timed steps (wall clock
 bracketed by device synchronizations).  `roofline` prices the dominant kernel at the
@@ -19,7 +20,9 @@ The JSON also carries the metric's two other forms: `fwd_bwd` (jittered level-6
 sphere, 2^20 points, loss = mean(udf^2), forward + hand-written backward) and
 `batched` (512 perturbed icospheres of 1,280..327,680 faces with 4,096..262,144
 uniform points each: one autotuned, HIP-graph-replayed forest launch against 512
-per-mesh calls).  --extras adds watertight SDF, ray tracing, raystab and floodfill.
+per-mesh calls), and `hashtable` (sparse-voxel hash table build and search, with the
+reference's CPU table timed beside it).  --extras adds watertight SDF, ray tracing,
+raystab, floodfill and the GPU tree build.
 """
 from __future__ import annotations
 
@@ -251,6 +254,63 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
             "tuning": list(tuning), "equal_to_per_mesh": ok}
 
 
+def hashtable_leg(dev, steps, warmup, with_cpu=True):
+    """Sparse-voxel hash table (cuHashTable): the distinct surface voxels of a sphere
+    at 1024^3 (from 2^23 seeded surface samples) inserted in random order, then 2^24
+    queries, half of them stored keys and half uniform voxels (mostly absent)."""
+    import torch
+
+    import cubvh_amd
+
+    gen = torch.Generator(device=dev).manual_seed(0)
+    g = torch.randn(1 << 23, 3, device=dev, generator=gen)
+    g = g / g.norm(dim=1, keepdim=True)
+    v = ((g * 0.45 + 0.5) * 1024).floor().to(torch.int64).clamp_(0, 1023)
+    key = torch.unique((v[:, 0] << 20) | (v[:, 1] << 10) | v[:, 2])
+    key = key[torch.randperm(key.numel(), device=dev, generator=gen)]
+    C = torch.stack([key >> 20, (key >> 10) & 1023, key & 1023], dim=1).to(torch.int32).contiguous()
+    n = C.shape[0]
+    m = 1 << 24
+    hits = C[torch.randint(0, n, (m // 2,), device=dev, generator=gen)]
+    miss = torch.randint(0, 1024, (m - m // 2, 3), device=dev, generator=gen, dtype=torch.int32)
+    Q = torch.cat([hits, miss])[torch.randperm(m, device=dev, generator=gen)].contiguous()
+    ht = cubvh_amd.cuHashTable()
+    ht.build(C)  # allocates the table; the timed builds reuse it
+    _, tb = _time_kernel(lambda: ht.impl.build(C), steps, warmup)
+    _, ts = _time_kernel(lambda: ht.impl.search(Q), steps, warmup)
+    found = int((ht.search(Q) >= 0).sum())
+    res = {"workload": f"{n} distinct sphere-surface voxels of a 1024^3 grid (random insertion order), {m} queries "
+                       f"(half stored keys)", "keys": n, "queries": m, "found": found,
+           "build_ms": tb * 1e3, "build_Mkeys_s": n / tb / 1e6, "search_ms": ts * 1e3, "search_Mq_s": m / ts / 1e6,
+           "search_GB_s_32B_per_query": 32 * m / ts / 1e9,
+           "search_frac_of_hbm_peak_32B_per_query": 32 * m / ts / 1e9 / HBM_PEAK_GBS}
+    if with_cpu:
+        from oracle import oracle as O
+
+        c_np = C.cpu().numpy()
+        q_np = Q[: 1 << 21].cpu().numpy()
+        kind = "reference" if O.ref_hash_available() else "port"
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_hash_build_search(c_np, q_np[:0])
+        else:
+            O.hash_build(c_np)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_hash_build_search(c_np, q_np)
+        else:
+            O.hash_lookup(c_np, q_np)
+        t_both = time.perf_counter() - t0
+        t_search = max(t_both - t_build, 1e-9)
+        res["cpu_baseline"] = {"kind": kind, "cores": 1, "build_Mkeys_s": n / t_build / 1e6,
+                               "search_Mq_s": len(q_np) / t_search / 1e6,
+                               "sample": f"all {n} keys, the first {len(q_np)} queries, one thread: the "
+                                         f"reference's CPU table (include/cpu/hashtable.h)"
+                                         if kind == "reference" else "the oracle's restatement, one thread"}
+    return res
+
+
 def run_extras(dev, steps, warmup, out):
     import torch
 
@@ -322,6 +382,8 @@ def main():
     ap.add_argument("--leaf", type=int, default=0, help="leaf_size argument of cuBVH (0 = library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device tensor copy first (reference for HBM counters)")
+    ap.add_argument("--hashtable-only", action="store_true",
+                    help="run only the sparse-voxel hash table leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
@@ -334,6 +396,17 @@ def main():
     from cubvh_amd import synthetic as S
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
+
+    if args.hashtable_only:
+        print(json.dumps(hashtable_leg(dev, args.steps, args.warmup, with_cpu=not args.no_cpu_baseline)), flush=True)
+        return
     v, f, meta = S.perturbed_sphere(8, seed=0)
     t0 = time.perf_counter()
     bvh = cubvh_amd.cuBVH(v, f, leaf_size=args.leaf)
@@ -347,14 +420,6 @@ def main():
     import ctypes
 
     stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-
-    if args.copy_calibration:
-        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)  # reads and writes exactly 1 GiB
-        torch.cuda.synchronize()
-        del src, dst
 
     order = torch.empty(n, dtype=torch.int32, device=dev)
 
@@ -420,6 +485,7 @@ def main():
     if not args.headline_only:
         result["fwd_bwd"] = fwd_bwd_leg(dev, max(3, args.steps), args.warmup)
         result["batched"] = batched_leg(dev, max(2, args.steps // 3), 1)
+        result["hashtable"] = hashtable_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
     extra = {}
     if args.extras:
         run_extras(dev, max(2, args.steps // 2), 1, extra)

@@ -1,8 +1,9 @@
 """cubvh_amd: the BVH query path of ashawkey/cubvh, MI355X-native (HIP, gfx950).
 
 Drop-in for the reference package's exports (cubvh/__init__.py:1-11): use
-``import cubvh_amd as cubvh``.  The BVH query path (cuBVH, floodfill) runs in
-libcubvh_amd.so; the other reference names raise NotImplementedError.
+``import cubvh_amd as cubvh``.  The BVH query path (cuBVH, floodfill) and the
+sparse-voxel hash table (cuHashTable) run in libcubvh_amd.so; the other reference
+names raise NotImplementedError.
 Additions: :func:`udf` (differentiable unsigned distance), :class:`cuBVHForest`
 (one launch over many meshes), :class:`ForestGraph` (HIP-graph replay of a forest
 query), :func:`udf_batched`, :func:`tune_forest` (launch autotuner),
@@ -12,7 +13,6 @@ many times).
 from .api import (
     cuBVH,
     HashTable,
-    cuHashTable,
     sparse_marching_cubes,
     sparse_marching_cubes_cpu,
     floodfill,
@@ -23,6 +23,7 @@ from .api import (
     morton_order,
 )
 from .autograd import udf
+from .hashtable import cuHashTable, create_cuHashTable
 from .forest import cuBVHForest, ForestGraph, udf_batched
 from .autotune import tune_forest
 
@@ -43,4 +44,5 @@ __all__ = [
     "udf_batched",
     "tune_forest",
     "morton_order",
+    "create_cuHashTable",
 ]

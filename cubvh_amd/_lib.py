@@ -58,6 +58,15 @@ SIGNATURES = {
         [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
          c_void_p, c_void_p],
     ),
+    "cubvh_hashtable_create": (c_int, [c_int32, POINTER(c_void_p)]),
+    "cubvh_hashtable_destroy": (None, [c_void_p]),
+    "cubvh_hashtable_set_num_dims": (c_int, [c_void_p, c_int32]),
+    "cubvh_hashtable_info": (c_int, [c_void_p, c_void_p]),
+    "cubvh_hashtable_resize": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
+    "cubvh_hashtable_prepare": (c_int, [c_void_p, c_void_p]),
+    "cubvh_hashtable_insert": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p]),
+    "cubvh_hashtable_build": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p]),
+    "cubvh_hashtable_search": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -16,6 +16,7 @@
 #include "bvh.hpp"
 #include "forest.hpp"
 #include "gpubuild.hpp"
+#include "hashtable.hpp"
 #include "query.hpp"
 
 using namespace cubvh_amd;
@@ -1044,6 +1045,216 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.n_cu = c.n_cu;
     e = launch_forest_distance(a, ws, hs);
     if (e != hipSuccess) return fail_hip(e, "forest kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+// ---------------------------------------------------------------------------
+// Sparse-voxel hash table (include/gpu/hashtable.cuh, src/api_gpu.cu:215-281).
+
+}  // extern "C"
+
+struct cubvh_hashtable {
+    std::mutex mu;
+    int32_t dims = 3;        // key words expected by the next insert / build
+    int32_t table_dims = 3;  // key words the table memory is laid out for
+    int64_t capacity = 0;
+    int device = -1;
+    int32_t* table = nullptr;
+    size_t bytes = 0;
+    int64_t rows = 0;  // rows inserted since the table was last cleared
+
+    // Memory for `cap` slots of keys of `d` words on `dev`, every slot empty.
+    int lay_out(int dev, int64_t cap, int32_t d, hipStream_t s) {
+        const size_t need = (size_t)cap * (size_t)hash_slot_words(d) * sizeof(int32_t);
+        DeviceScope g(dev);
+        if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+        if (table && (device != dev || bytes < need)) {
+            int rc = release();
+            if (rc != CUBVH_OK) return rc;
+        }
+        if (!table && need > 0) {
+            hipError_t e = hipMalloc((void**)&table, need);
+            if (e != hipSuccess) {
+                table = nullptr;
+                return fail_hip(e, "hipMalloc (hash table)");
+            }
+            bytes = need;
+        }
+        device = dev;
+        capacity = cap;
+        table_dims = d;
+        rows = 0;
+        hipError_t e = hash_clear(table, cap, hash_slot_words(d), s);
+        if (e != hipSuccess) return fail_hip(e, "hash table clear");
+        return CUBVH_OK;
+    }
+    int release() {
+        if (!table) return CUBVH_OK;
+        DeviceScope g(device);
+        hipError_t e = g.err == hipSuccess ? hipFree(table) : g.err;
+        table = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) return fail_hip(e, "hipFree (hash table)");
+        return CUBVH_OK;
+    }
+    int check_device(int dev) const {
+        if (table && dev != device)
+            return fail(CUBVH_ERR_ARG, "the hash table lives on device " + std::to_string(device) +
+                                           ", the keys on device " + std::to_string(dev));
+        return CUBVH_OK;
+    }
+    int insert(int dev, const int32_t* coords, int64_t n, int32_t d, hipStream_t s) {
+        if (capacity <= 0 || n <= 0) {  // as the reference: nothing to insert
+            dims = d;
+            return CUBVH_OK;
+        }
+        if (!coords) return fail(CUBVH_ERR_ARG, "null coords buffer");
+        if (n > (int64_t)INT32_MAX) return fail(CUBVH_ERR_ARG, "at most 2^31 - 1 rows per insert");
+        int rc = check_device(dev);
+        if (rc != CUBVH_OK) return rc;
+        if (d != table_dims && rows > 0)
+            return fail(CUBVH_ERR_ARG, "the table holds keys of " + std::to_string(table_dims) +
+                                           " words; prepare() it before inserting keys of " + std::to_string(d));
+        dims = d;
+        if (d != table_dims) {
+            rc = lay_out(dev, capacity, d, s);
+            if (rc != CUBVH_OK) return rc;
+        }
+        DeviceScope g(dev);
+        if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+        hipError_t e = hash_insert(table, capacity, hash_slot_words(d), coords, n, d, s);
+        if (e != hipSuccess) return fail_hip(e, "hash insert kernel launch");
+        rows += n;
+        return CUBVH_OK;
+    }
+};
+
+namespace {
+int check_dims(int32_t d) {
+    if (d < 1 || d > kHashMaxDims)
+        return fail(CUBVH_ERR_ARG, "keys must have 1.." + std::to_string(kHashMaxDims) + " words, got " +
+                                       std::to_string(d));
+    return CUBVH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cubvh_hashtable_create(int32_t num_dims, cubvh_hashtable** out) {
+    CUBVH_GUARD_BEGIN
+    if (!out) return fail(CUBVH_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = check_dims(num_dims);
+    if (rc != CUBVH_OK) return rc;
+    cubvh_hashtable* h = new cubvh_hashtable();
+    h->dims = h->table_dims = num_dims;
+    *out = h;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+void cubvh_hashtable_destroy(cubvh_hashtable* ht) {
+    if (!ht) return;
+    (void)ht->release();
+    delete ht;
+}
+
+int cubvh_hashtable_set_num_dims(cubvh_hashtable* ht, int32_t num_dims) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    int rc = check_dims(num_dims);
+    if (rc != CUBVH_OK) return rc;
+    std::lock_guard<std::mutex> lock(ht->mu);
+    ht->dims = num_dims;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_info(cubvh_hashtable* ht, int64_t info[5]) {
+    CUBVH_GUARD_BEGIN
+    if (!ht || !info) return fail(CUBVH_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(ht->mu);
+    info[0] = ht->dims;
+    info[1] = ht->capacity;
+    info[2] = ht->table_dims;
+    info[3] = hash_slot_words(ht->table_dims);
+    info[4] = ht->table ? ht->device : -1;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_resize(cubvh_hashtable* ht, int device, int64_t capacity, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    if (capacity < 0 || capacity > (int64_t)INT32_MAX) return fail(CUBVH_ERR_ARG, "capacity must be in [0, 2^31 - 1]");
+    std::lock_guard<std::mutex> lock(ht->mu);
+    if (capacity == 0) {
+        ht->capacity = 0;
+        ht->rows = 0;
+        return ht->release();
+    }
+    return ht->lay_out(device, capacity, ht->dims, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_prepare(cubvh_hashtable* ht, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    std::lock_guard<std::mutex> lock(ht->mu);
+    if (ht->capacity <= 0) return CUBVH_OK;  // as the reference: nothing to clear
+    return ht->lay_out(ht->device, ht->capacity, ht->dims, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_insert(cubvh_hashtable* ht, int device, const int32_t* coords, int64_t n, int32_t num_dims,
+                           void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative row count");
+    int rc = check_dims(num_dims);
+    if (rc != CUBVH_OK) return rc;
+    std::lock_guard<std::mutex> lock(ht->mu);
+    return ht->insert(device, coords, n, num_dims, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_build(cubvh_hashtable* ht, int device, const int32_t* coords, int64_t n, int32_t num_dims,
+                          void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative row count");
+    if (n > ((int64_t)INT32_MAX - 1) / 2) return fail(CUBVH_ERR_ARG, "at most 2^30 - 1 rows per build");
+    if (n > 0 && !coords) return fail(CUBVH_ERR_ARG, "null coords buffer");
+    int rc = check_dims(num_dims);
+    if (rc != CUBVH_OK) return rc;
+    std::lock_guard<std::mutex> lock(ht->mu);
+    ht->dims = num_dims;
+    rc = ht->lay_out(device, std::max<int64_t>(16, 2 * n), num_dims, (hipStream_t)stream);
+    if (rc != CUBVH_OK) return rc;
+    return ht->insert(device, coords, n, num_dims, (hipStream_t)stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_hashtable_search(cubvh_hashtable* ht, int device, const int32_t* queries, int64_t m, int32_t num_dims,
+                           int32_t* out, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!ht) return fail(CUBVH_ERR_ARG, "null hash table");
+    if (m < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    std::lock_guard<std::mutex> lock(ht->mu);
+    if (ht->capacity <= 0 || !ht->table) return fail(CUBVH_ERR_ARG, "hash table is not built");
+    if (num_dims != ht->table_dims)
+        return fail(CUBVH_ERR_ARG, "queries have " + std::to_string(num_dims) + " words, the table's keys " +
+                                       std::to_string(ht->table_dims));
+    if (m == 0) return CUBVH_OK;
+    if (!queries || !out) return fail(CUBVH_ERR_ARG, "null query or output buffer");
+    int rc = ht->check_device(device);
+    if (rc != CUBVH_OK) return rc;
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    hipError_t e = hash_search(ht->table, ht->capacity, hash_slot_words(num_dims), queries, m, num_dims, out,
+                               (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "hash search kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END
 }

@@ -208,6 +208,56 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
                                    const int64_t* query_offsets, float* distances, int64_t* face_id, float* uvw,
                                    const int32_t* tuning, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Sparse-voxel hash table: a static open-addressed table of int32 keys of D words
+ * (1 <= D <= 63), linear probing.  Replaces the reference's cuHashTable
+ * (include/gpu/api_gpu.h:37-52, src/api_gpu.cu:215-281, include/gpu/hashtable.cuh,
+ * bound at src/bindings.cpp:34-44).
+ * Differences, all on purpose: the table stores the keys (the reference keeps a
+ * pointer to the caller's coords, so its table breaks when they are freed or when a
+ * second insert() replaces them); duplicate keys answer with their lowest row index
+ * (the reference's GPU table answers with any one of them, its CPU table with the
+ * lowest); capacities and key widths are checked.
+ * ------------------------------------------------------------------------- */
+typedef struct cubvh_hashtable cubvh_hashtable;
+
+/* Empty table for keys of num_dims words (capacity 0: nothing allocated).
+ * Replaces create_cuHashTable() + set_num_dims(d)   src/api_gpu.cu:278-280. */
+int cubvh_hashtable_create(int32_t num_dims, cubvh_hashtable** out);
+/* Frees the table's device memory (synchronizes its device). */
+void cubvh_hashtable_destroy(cubvh_hashtable* ht);
+
+/* Key width expected by the next insert.   Replaces set_num_dims   api_gpu.cu:222-224. */
+int cubvh_hashtable_set_num_dims(cubvh_hashtable* ht, int32_t num_dims);
+/* info[0] = num_dims (get_num_dims, api_gpu.cu:226-228), info[1] = capacity (slots),
+ * info[2] = key words the table memory holds, info[3] = int32 words per slot,
+ * info[4] = device of the table memory (-1 if none). */
+int cubvh_hashtable_info(cubvh_hashtable* ht, int64_t info[5]);
+
+/* capacity slots on `device`, all empty (0 frees the table).  Reallocates only when
+ * the current memory is too small or on another device (hipMalloc / hipFree).
+ * Replaces resize(capacity)   api_gpu.cu:230-232 (whose slots stay uninitialised). */
+int cubvh_hashtable_resize(cubvh_hashtable* ht, int device, int64_t capacity, void* stream);
+/* Marks every slot empty (laid out for the current num_dims); no-op at capacity 0.
+ * Replaces prepare()   api_gpu.cu:234-237, hashtable.cuh:124-131,181-186. */
+int cubvh_hashtable_prepare(cubvh_hashtable* ht, void* stream);
+/* Inserts the n rows of coords int32 [n, num_dims] (device memory on `device`), row i
+ * with index i; sets num_dims.  No-op at capacity 0 or n == 0.  A row that finds no
+ * free slot (table full) is dropped, as in the reference.  Keys of another width than
+ * the ones already stored are refused until prepare().
+ * Replaces insert(coords)   api_gpu.cu:239-249, hashtable.cuh:136-158,188-195. */
+int cubvh_hashtable_insert(cubvh_hashtable* ht, int device, const int32_t* coords, int64_t n, int32_t num_dims,
+                           void* stream);
+/* capacity = max(16, 2 n), clear, insert (n < 2^30).
+ * Replaces build(coords)   api_gpu.cu:251-261, hashtable.cuh:197-206. */
+int cubvh_hashtable_build(cubvh_hashtable* ht, int device, const int32_t* coords, int64_t n, int32_t num_dims,
+                          void* stream);
+/* out int32 [m]: the lowest row index stored with key queries[i] (int32 [m, num_dims],
+ * num_dims equal to the stored keys' width), or -1.  Fails on a table never built.
+ * Replaces search(queries)   api_gpu.cu:263-274, hashtable.cuh:161-176,208-214. */
+int cubvh_hashtable_search(cubvh_hashtable* ht, int device, const int32_t* queries, int64_t m, int32_t num_dims,
+                           int32_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

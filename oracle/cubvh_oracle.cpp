@@ -51,6 +51,15 @@
 //      (include/gpu/floodfill.cuh:14-272, cubic grids) and a BFS over tensor-order
 //      6-connectivity.  Pinned against scipy.ndimage.label and hand-built grids.
 //
+//  (7) the sparse-voxel hash table (include/cpu/hashtable.h:20-147, the sequential
+//      form of include/gpu/hashtable.cuh:24-217): (marker, row) slots, the
+//      CityHash-like slot function of the reference (kept alike so that the table can
+//      be compared slot for slot with the reference's own CPU table), linear probing,
+//      rows inserted in order.  Pinned against the known answers of the reference's
+//      test/hashtable.py, against a brute-force dictionary (the lowest row index of
+//      each key), and slot for slot against the reference's CPU table compiled from
+//      its header (oracle/_ref, when the reference sources are present).
+//
 // sin/cos of the stab directions are evaluated as (float)sin((double)x): a libm
 // result, not the CUDA math library.
 
@@ -1272,6 +1281,72 @@ int orc_floodfill_bfs(const uint8_t* grid, int B, int H, int W, int D, int32_t* 
         }
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// (7) hash table.  Slot function: include/cpu/hashtable.h:21-42.
+static uint32_t hash_step(uint32_t h, uint32_t k) {
+    h += k * 0x9E3779B9u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+static int hash_home(const int32_t* key, int d, int capacity) {
+    uint32_t h = 0;
+    for (int i = 0; i < d; ++i) h = hash_step(h, (uint32_t)key[i]);
+    int slot = (int)h % capacity;  // signed remainder, as the reference
+    return slot < 0 ? slot + capacity : slot;
+}
+
+// Clears table[2 * capacity] to -1 and inserts rows 0..n-1 of coords [n, d] in order
+// (include/cpu/hashtable.h:97-124, build: 126-132 with capacity max(16, 2n)).
+// Returns the number of rows that found no slot.
+int64_t orc_hash_build(const int32_t* coords, int64_t n, int d, int capacity, int32_t* table) {
+    std::fill(table, table + 2 * (int64_t)capacity, -1);
+    int64_t dropped = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int slot = hash_home(coords + i * d, d, capacity);
+        const int begin = slot;
+        bool placed = false;
+        for (int a = 0; a < capacity; ++a) {
+            if (table[2 * slot] == -1) {
+                table[2 * slot] = slot;
+                table[2 * slot + 1] = (int32_t)i;
+                placed = true;
+                break;
+            }
+            slot = (slot + 1) % capacity;
+            if (slot == begin) break;
+        }
+        dropped += placed ? 0 : 1;
+    }
+    return dropped;
+}
+
+// out[i] = row index of the first stored key equal to queries[i] along its probe run,
+// -1 at the first empty slot or after a full cycle (include/cpu/hashtable.h:58-86).
+void orc_hash_search(const int32_t* table, int capacity, const int32_t* coords, int d, const int32_t* queries,
+                     int64_t m, int32_t* out) {
+    for (int64_t i = 0; i < m; ++i) {
+        const int32_t* q = queries + i * d;
+        int slot = hash_home(q, d, capacity);
+        const int begin = slot;
+        int32_t r = -1;
+        for (int a = 0; a < capacity; ++a) {
+            if (table[2 * slot] == -1) break;
+            const int32_t row = table[2 * slot + 1];
+            if (row != -1 && std::equal(q, q + d, coords + (int64_t)row * d)) {
+                r = row;
+                break;
+            }
+            slot = (slot + 1) % capacity;
+            if (slot == begin) break;
+        }
+        out[i] = r;
+    }
 }
 
 }  // extern "C"

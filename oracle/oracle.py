@@ -18,13 +18,23 @@ HERE = Path(__file__).resolve().parent
 _lib = None
 
 
+REFERENCE = Path("/root/reference")
+REF_HASH_LIB = HERE / "_ref" / "libref_hashtable.so"
+
+
 def build(verbose: bool = False) -> None:
-    """Compile both oracle variants with the committed Makefile (g++)."""
+    """Compile both oracle variants with the committed Makefile (g++), and the
+    reference's own CPU hash table into _ref/ when the reference sources are present."""
     r = subprocess.run(["make", "-B", "-C", str(HERE)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout)
     if verbose:
         print(r.stdout)
+    if (REFERENCE / "include" / "cpu" / "hashtable.h").exists():
+        r = subprocess.run(["make", "-B", "-C", str(HERE), "ref", f"REF={REFERENCE}"], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference hash table build failed:\n" + r.stdout)
 
 
 def _cpu_has_fma() -> bool:
@@ -73,6 +83,8 @@ def load():
         "orc_refbvh_sdf": (c_int, [vp, vp, i64, c_uint32, vp, vp, vp, vp, c_int]),
         "orc_floodfill_ref": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
         "orc_floodfill_bfs": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
+        "orc_hash_build": (i64, [vp, i64, c_int, c_int, vp]),
+        "orc_hash_search": (None, [vp, c_int, vp, c_int, vp, i64, vp]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -299,3 +311,60 @@ def floodfill_bfs(grid):
     out = np.empty(g.shape, np.int32)
     load().orc_floodfill_bfs(_p(g), B, H, W, D, _p(out))
     return out
+
+
+# ---------------------------------------------------------------- hash table
+def hash_build(coords, capacity=None):
+    """The reference's sequential table over coords int32 [n, d]: (marker, row) int32
+    [capacity, 2] and the number of rows that found no slot.  capacity None: the
+    reference's build() size max(16, 2n)."""
+    c = _i32(coords)
+    n, d = c.shape
+    cap = max(16, 2 * n) if capacity is None else int(capacity)
+    table = np.empty((cap, 2), np.int32)
+    dropped = load().orc_hash_build(_p(c), n, d, cap, _p(table))
+    return table, int(dropped)
+
+
+def hash_search(table, coords, queries):
+    """int32 [m]: the first stored row equal to each query along its probe run, or -1."""
+    c = _i32(coords)
+    q = _i32(queries)
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    out = np.empty(q.shape[0], np.int32)
+    load().orc_hash_search(_p(t), t.shape[0], _p(c), q.shape[1], _p(q), q.shape[0], _p(out))
+    return out
+
+
+def hash_lookup(coords, queries, capacity=None):
+    """build + search in one call."""
+    table, _ = hash_build(coords, capacity)
+    return hash_search(table, coords, queries)
+
+
+_ref_hash = None
+
+
+def ref_hash_available() -> bool:
+    return REF_HASH_LIB.exists()
+
+
+def ref_hash_build_search(coords, queries, capacity=None):
+    """The reference's own CPU table (include/cpu/hashtable.h, built into _ref/):
+    (search results int32 [m], table int32 [capacity, 2])."""
+    global _ref_hash
+    if _ref_hash is None:
+        lib = ctypes.CDLL(str(REF_HASH_LIB))
+        lib.ref_hash_build_search.restype = c_int
+        lib.ref_hash_build_search.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]
+        _ref_hash = lib
+    c = _i32(coords)
+    q = _i32(queries)
+    n, d = c.shape
+    cap = max(16, 2 * n) if capacity is None else int(capacity)
+    table = np.empty((cap, 2), np.int32)
+    out = np.empty(q.shape[0], np.int32)
+    got = _ref_hash.ref_hash_build_search(_p(c), n, d, 0 if capacity is None else cap, _p(q), q.shape[0], _p(out),
+                                          _p(table))
+    assert got == cap
+    return out, table

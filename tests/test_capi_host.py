@@ -191,7 +191,41 @@ def test_reference_package_names_exist():
     with pytest.raises(NotImplementedError):
         cubvh_amd.fill_holes(None, None)
     with pytest.raises(NotImplementedError):
-        cubvh_amd.cuHashTable()
+        cubvh_amd.HashTable()
+
+
+def test_hashtable_host_side_checks():
+    """cuHashTable's host logic: dims, argument checks (src/api_gpu.cu:239-274), and a
+    search before any build fails; nothing here touches a GPU."""
+    import cubvh_amd
+
+    ht = cubvh_amd.cuHashTable()
+    assert ht.num_dims == 3
+    ht.impl.set_num_dims(5)
+    assert ht.num_dims == 5 and ht.impl.capacity == 0
+    with pytest.raises(cubvh_amd._lib.CubvhError, match="1..63"):
+        ht.impl.set_num_dims(0)
+    with pytest.raises(cubvh_amd._lib.CubvhError, match="1..63"):
+        cubvh_amd.cuHashTable(num_dims=64)
+    cpu = torch.zeros(4, 3, dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="reside on CUDA"):
+        ht.impl.build(cpu)
+    with pytest.raises(RuntimeError, match="reside on CUDA"):
+        ht.impl.search(cpu)
+    lib = cubvh_amd._lib.load()
+    h = ctypes.c_void_p()
+    cubvh_amd._lib.check(lib.cubvh_hashtable_create(3, ctypes.byref(h)))
+    try:
+        assert lib.cubvh_hashtable_search(h, 0, None, 4, 3, None, None) == -1
+        assert "not built" in lib.cubvh_last_error().decode()
+        assert lib.cubvh_hashtable_resize(h, 0, -1, None) == -1
+        assert lib.cubvh_hashtable_insert(h, 0, None, -1, 3, None) == -1
+        # capacity 0: insert and prepare do nothing, as in the reference
+        assert lib.cubvh_hashtable_insert(h, 0, None, 10, 3, None) == 0
+        assert lib.cubvh_hashtable_prepare(h, None) == 0
+        assert lib.cubvh_hashtable_build(h, 0, None, 1 << 30, 3, None) == -1
+    finally:
+        lib.cubvh_hashtable_destroy(h)
 
 
 def test_floodfill_requires_bool_grid():
