@@ -7,7 +7,7 @@ points (cubvh_morton_order), then walk them (cubvh_distance_ordered) -- the two 
 of cubvh_unsigned_distance, whose own rate is reported as config.dropin_call_Mq_s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline] [--headline-only]
-                 [--hashtable-only]
+                 [--hashtable-only] [--smc-only]
 
 Prints one JSON line.  `value` = M queries/s over the timed steps (wall clock
 bracketed by device synchronizations).  `roofline` prices the dominant kernel at the
@@ -20,8 +20,9 @@ The JSON also carries the metric's two other forms: `fwd_bwd` (jittered level-6
 sphere, 2^20 points, loss = mean(udf^2), forward + hand-written backward) and
 `batched` (512 perturbed icospheres of 1,280..327,680 faces with 4,096..262,144
 uniform points each: one autotuned, HIP-graph-replayed forest launch against 512
-per-mesh calls), and `hashtable` (sparse-voxel hash table build and search, with the
-reference's CPU table timed beside it).  --extras adds watertight SDF, ray tracing,
+per-mesh calls), `hashtable` (sparse-voxel hash table build and search) and
+`sparse_marching_cubes` (the cut cells of a 1024^3 lattice), each with the reference's
+CPU implementation timed beside it.  --extras adds watertight SDF, ray tracing,
 raystab, floodfill and the GPU tree build.
 """
 from __future__ import annotations
@@ -311,6 +312,71 @@ def hashtable_leg(dev, steps, warmup, with_cpu=True):
     return res
 
 
+def smc_cells(dev, res=1024):
+    """Cells of a res^3 lattice cut by a bumpy sphere (the analytic field
+    |p - c| - 0.42 res - 6 sin(x/40) sin(y/40) sin(z/40) in lattice units), selected as
+    the reference's remesh script selects them (corners of both signs), with their 8
+    corner values in the reference's corner order."""
+    import torch
+
+    ax = torch.arange(res + 1, device=dev, dtype=torch.float32)
+    c = res / 2 + 0.3
+    x, y, z = ax.view(-1, 1, 1), ax.view(1, -1, 1), ax.view(1, 1, -1)
+    f = torch.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) - 0.42 * res
+    f -= 6.0 * torch.sin(x / 40) * torch.sin(y / 40) * torch.sin(z / 40)
+    inside = f < 0
+    offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    shifted = [inside[a:a + res, b:b + res, d:d + res] for a, b, d in offs]
+    some = shifted[0].clone()
+    every = shifted[0].clone()
+    for t in shifted[1:]:
+        some |= t
+        every &= t
+    del shifted, inside
+    idx = torch.nonzero(some & ~every)
+    corners = torch.stack([f[idx[:, 0] + a, idx[:, 1] + b, idx[:, 2] + d] for a, b, d in offs], dim=1)
+    del f
+    return idx.to(torch.int32).contiguous(), corners.contiguous()
+
+
+def smc_leg(dev, steps, warmup, with_cpu=True):
+    """Sparse marching cubes over the cut cells of a 1024^3 lattice; one step is one
+    cubvh_amd.sparse_marching_cubes call (inputs resident on the GPU, outputs allocated
+    by the call, as in the reference's API)."""
+    import torch
+
+    import cubvh_amd
+
+    C, V = smc_cells(dev)
+    n = C.shape[0]
+    wall, _ = _time_kernel(lambda: cubvh_amd.sparse_marching_cubes(C, V, 0.0), steps, warmup)
+    verts, tris = cubvh_amd.sparse_marching_cubes(C, V, 0.0)
+    wall_c, _ = _time_kernel(lambda: cubvh_amd.sparse_marching_cubes(C, V, 0.0, ensure_consistency=True),
+                             max(2, steps // 2), 1)
+    res = {"workload": f"{n} cut cells of a 1024^3 lattice (bumpy sphere), iso 0", "cells": n,
+           "vertices": int(verts.shape[0]), "triangles": int(tris.shape[0]),
+           "ms": wall / steps * 1e3, "Mcells_s": n * steps / wall / 1e6,
+           "ensure_consistency_ms": wall_c / max(2, steps // 2) * 1e3}
+    if with_cpu:
+        from oracle import oracle as O
+
+        sample = 1 << 20
+        c_np = C[:sample].cpu().numpy()
+        v_np = V[:sample].cpu().numpy()
+        kind = "reference" if O.ref_smc_available() else "port"
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_smc_cpu(c_np, v_np, 0.0)
+        else:
+            O.smc(c_np, v_np, 0.0)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"kind": kind, "cores": 1, "Mcells_s": len(c_np) / t / 1e6,
+                               "sample": f"the first {len(c_np)} cells, one thread: "
+                                         + ("the reference's CPU extraction (include/cpu/spmc.h)"
+                                            if kind == "reference" else "the oracle's restatement")}
+    return res
+
+
 def run_extras(dev, steps, warmup, out):
     import torch
 
@@ -384,6 +450,8 @@ def main():
                     help="run a 1 GiB device-to-device tensor copy first (reference for HBM counters)")
     ap.add_argument("--hashtable-only", action="store_true",
                     help="run only the sparse-voxel hash table leg and print its JSON")
+    ap.add_argument("--smc-only", action="store_true",
+                    help="run only the sparse marching cubes leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
@@ -406,6 +474,9 @@ def main():
 
     if args.hashtable_only:
         print(json.dumps(hashtable_leg(dev, args.steps, args.warmup, with_cpu=not args.no_cpu_baseline)), flush=True)
+        return
+    if args.smc_only:
+        print(json.dumps(smc_leg(dev, args.steps, args.warmup, with_cpu=not args.no_cpu_baseline)), flush=True)
         return
     v, f, meta = S.perturbed_sphere(8, seed=0)
     t0 = time.perf_counter()
@@ -486,6 +557,7 @@ def main():
         result["fwd_bwd"] = fwd_bwd_leg(dev, max(3, args.steps), args.warmup)
         result["batched"] = batched_leg(dev, max(2, args.steps // 3), 1)
         result["hashtable"] = hashtable_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
+        result["sparse_marching_cubes"] = smc_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
     extra = {}
     if args.extras:
         run_extras(dev, max(2, args.steps // 2), 1, extra)

@@ -67,6 +67,13 @@ SIGNATURES = {
     "cubvh_hashtable_insert": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p]),
     "cubvh_hashtable_build": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p]),
     "cubvh_hashtable_search": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "cubvh_sparse_marching_cubes": (
+        c_int,
+        [c_int, c_void_p, c_void_p, c_int64, c_float, c_int32, POINTER(c_void_p), POINTER(c_int64), POINTER(c_int64),
+         c_void_p],
+    ),
+    "cubvh_smc_write": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "cubvh_smc_destroy": (None, [c_void_p]),
 }
 
 _lib = None

@@ -297,7 +297,6 @@ class HashTable(_OutOfScopeClass):
     _name = "HashTable"
 
 
-sparse_marching_cubes = _out_of_scope("sparse_marching_cubes")
 sparse_marching_cubes_cpu = _out_of_scope("sparse_marching_cubes_cpu")
 fill_holes = _out_of_scope("fill_holes")
 merge_vertices = _out_of_scope("merge_vertices")

@@ -17,6 +17,7 @@
 #include "forest.hpp"
 #include "gpubuild.hpp"
 #include "hashtable.hpp"
+#include "smc.hpp"
 #include "query.hpp"
 
 using namespace cubvh_amd;
@@ -1257,6 +1258,76 @@ int cubvh_hashtable_search(cubvh_hashtable* ht, int device, const int32_t* queri
     if (e != hipSuccess) return fail_hip(e, "hash search kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END
+}
+
+// ---------------------------------------------------------------------------
+// Sparse marching cubes (include/gpu/spcumc.cuh:631-868, src/api_gpu.cu:154-211).
+
+}  // extern "C"
+
+struct cubvh_smc {
+    SmcState* st = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_vertices = 0, n_triangles = 0;
+    bool written = false;
+};
+
+extern "C" {
+
+int cubvh_sparse_marching_cubes(int device, const int32_t* coords, const float* corners, int64_t n, float iso,
+                                int32_t ensure_consistency, cubvh_smc** out, int64_t* n_vertices,
+                                int64_t* n_triangles, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!out || !n_vertices || !n_triangles) return fail(CUBVH_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    *n_vertices = *n_triangles = 0;
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative cell count");
+    if (n > kSmcMaxCells) return fail(CUBVH_ERR_ARG, "at most 2^28 cells per extraction");
+    if (n > 0 && (!coords || !corners)) return fail(CUBVH_ERR_ARG, "null coords or corners buffer");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    int rc = ensure_pool(device);
+    if (rc != CUBVH_OK) return rc;
+    SmcState* st = nullptr;
+    int64_t nv = 0, nt = 0;
+    SmcStatus r = smc_extract(coords, corners, n, iso, ensure_consistency != 0, (hipStream_t)stream, &st, &nv, &nt);
+    if (r.code != 0) return fail(r.code == -1 ? CUBVH_ERR_ARG : CUBVH_ERR_HIP, r.message);
+    cubvh_smc* h = new cubvh_smc();
+    h->st = st;
+    h->device = device;
+    h->stream = (hipStream_t)stream;
+    h->n_vertices = nv;
+    h->n_triangles = nt;
+    *out = h;
+    *n_vertices = nv;
+    *n_triangles = nt;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+int cubvh_smc_write(cubvh_smc* smc, float* vertices, int32_t* triangles) {
+    CUBVH_GUARD_BEGIN
+    if (!smc || !smc->st) return fail(CUBVH_ERR_ARG, "null or released extraction");
+    if (smc->written) return fail(CUBVH_ERR_ARG, "the extraction was already written");
+    if ((smc->n_vertices > 0 && !vertices) || (smc->n_triangles > 0 && !triangles))
+        return fail(CUBVH_ERR_ARG, "null output buffer");
+    DeviceScope g(smc->device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    SmcStatus r = smc_write(smc->st, vertices, triangles, smc->stream);
+    if (r.code != 0) return fail(CUBVH_ERR_HIP, r.message);
+    smc->written = true;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
+void cubvh_smc_destroy(cubvh_smc* smc) {
+    if (!smc) return;
+    if (smc->st) {
+        DeviceScope g(smc->device);
+        smc_release(smc->st, smc->stream);
+    }
+    delete smc;
 }
 
 }  // extern "C"

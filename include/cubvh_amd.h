@@ -258,6 +258,35 @@ int cubvh_hashtable_build(cubvh_hashtable* ht, int device, const int32_t* coords
 int cubvh_hashtable_search(cubvh_hashtable* ht, int device, const int32_t* queries, int64_t m, int32_t num_dims,
                            int32_t* out, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Sparse marching cubes.  Replaces sparse_marching_cubes(coords, corners, iso,
+ * ensure_consistency)   src/api_gpu.cu:154-211, include/gpu/spcumc.cuh:631-868
+ * (bound at src/bindings.cpp:28-31), in two calls so that the caller allocates the
+ * outputs once their sizes are known.
+ * Output (the reference's): one vertex per lattice edge with a sign change, in
+ * ascending (x, y, z, axis) order of the edge's lowest lattice point, placed by linear
+ * interpolation from the first cell (input order) holding the edge; per cell in input
+ * order, the triangles of its case (corner value < iso = inside) with the second and
+ * third vertex exchanged.  ensure_consistency replaces each corner value by the mean
+ * over the cells sharing the lattice point, summed in double in cell order (the
+ * reference's CPU path; its GPU path sums in fp32 in an unspecified order).
+ * ------------------------------------------------------------------------- */
+typedef struct cubvh_smc cubvh_smc;
+
+/* coords int32 [n, 3] (lowest lattice point of each cell), corners fp32 [n, 8] (corner
+ * k at offset ((k ^ k >> 1) & 1, k >> 1 & 1, k >> 2)), both on `device`, valid until
+ * cubvh_smc_write has been enqueued.  n <= 2^28; the cells may span at most 2^20 - 2
+ * lattice steps per axis.  Runs the extraction up to the vertex and triangle counts
+ * (synchronizes the stream) and returns a handle holding its scratch. */
+int cubvh_sparse_marching_cubes(int device, const int32_t* coords, const float* corners, int64_t n, float iso,
+                                int32_t ensure_consistency, cubvh_smc** out, int64_t* n_vertices,
+                                int64_t* n_triangles, void* stream);
+/* Writes vertices fp32 [n_vertices, 3] and triangles int32 [n_triangles, 3] (device
+ * buffers on the extraction's device) on the extraction's stream; once per handle. */
+int cubvh_smc_write(cubvh_smc* smc, float* vertices, int32_t* triangles);
+/* Releases the scratch (stream-ordered, after the writes) and the handle. */
+void cubvh_smc_destroy(cubvh_smc* smc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,17 @@
 //      each key), and slot for slot against the reference's CPU table compiled from
 //      its header (oracle/_ref, when the reference sources are present).
 //
+//  (8) sparse marching cubes: the output contract of the reference's GPU extraction
+//      (include/gpu/spcumc.cuh:379-530, 631-868) restated sequentially -- edge vertices
+//      keyed by (lowest lattice point, axis), ordered by a stable sort, placed from
+//      the first cell holding the edge; triangles per cell in table order with the
+//      second and third vertex exchanged -- and the corner averaging of its CPU path
+//      (include/cpu/spmc.h:408-433, double sums in cell order).  The case table is the
+//      product's (cubvh_amd/csrc/mc_table.hpp), pinned entry by entry against the
+//      reference's own; the rest is pinned against the reference's CPU extraction
+//      compiled from its header (same triangles and vertex positions, vertex order
+//      aside) and against closed-surface invariants.
+//
 // sin/cos of the stab directions are evaluated as (float)sin((double)x): a libm
 // result, not the CUDA math library.
 
@@ -70,8 +81,12 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
 #include <stack>
+#include <tuple>
 #include <vector>
+
+#include "../cubvh_amd/csrc/mc_table.hpp"  // the case table (pinned against the reference's)
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -1347,6 +1362,108 @@ void orc_hash_search(const int32_t* table, int capacity, const int32_t* coords, 
         }
         out[i] = r;
     }
+}
+
+// ---------------------------------------------------------------------------
+// (8) sparse marching cubes.
+// Cell corner offsets and edge end corners: include/gpu/spcumc.cuh:322-333.
+static const int kMcCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+static const int kMcEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                   {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// Vertices (at most 12 n) and triangles (at most 5 n) of the extraction; returns 0.
+int orc_smc(const int32_t* coords, const float* corners, int64_t n, float iso, int consistent, float* verts,
+            int32_t* tris, int64_t* n_verts, int64_t* n_tris) {
+    std::vector<float> val(corners, corners + 8 * n);
+    if (consistent) {
+        std::map<std::array<int64_t, 3>, std::pair<double, int>> acc;
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 8; ++k) {
+                auto& a = acc[{(int64_t)coords[3 * i] + kMcCorner[k][0], (int64_t)coords[3 * i + 1] + kMcCorner[k][1],
+                               (int64_t)coords[3 * i + 2] + kMcCorner[k][2]}];
+                a.first += (double)corners[8 * i + k];
+                a.second += 1;
+            }
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 8; ++k) {
+                const auto& a = acc[{(int64_t)coords[3 * i] + kMcCorner[k][0], (int64_t)coords[3 * i + 1] + kMcCorner[k][1],
+                                     (int64_t)coords[3 * i + 2] + kMcCorner[k][2]}];
+                val[8 * i + k] = (float)(a.first / a.second);
+            }
+    }
+    struct Raw {
+        std::array<int64_t, 4> key;
+        float p[3];
+    };
+    std::vector<Raw> raw;
+    std::vector<int64_t> first(n + 1, 0);
+    std::vector<int> cases(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* v = &val[8 * i];
+        int c = 0;
+        for (int k = 0; k < 8; ++k)
+            if (v[k] < iso) c |= 1 << k;
+        cases[i] = c;
+        first[i] = (int64_t)raw.size();
+        for (int e = 0; e < 12; ++e) {
+            const int a = kMcEdge[e][0], b = kMcEdge[e][1];
+            if (((c >> a) & 1) == ((c >> b) & 1)) continue;
+            Raw r;
+            for (int ax = 0; ax < 3; ++ax)
+                r.key[ax] = (int64_t)coords[3 * i + ax] + std::min(kMcCorner[a][ax], kMcCorner[b][ax]);
+            r.key[3] = kMcCorner[a][0] != kMcCorner[b][0] ? 0 : (kMcCorner[a][1] != kMcCorner[b][1] ? 1 : 2);
+            const float va = v[a], vb = v[b], denom = vb - va;
+            float t;
+            if (std::fabs(denom) < 1e-30f) {
+                t = 0.5f;
+            } else {
+                t = (iso - va) / denom;
+                t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+            }
+            for (int ax = 0; ax < 3; ++ax) {
+                const float pa = (float)((int64_t)coords[3 * i + ax] + kMcCorner[a][ax]);
+                const float pb = (float)((int64_t)coords[3 * i + ax] + kMcCorner[b][ax]);
+                r.p[ax] = pa + t * (pb - pa);
+            }
+            raw.push_back(r);
+        }
+    }
+    first[n] = (int64_t)raw.size();
+    std::vector<int64_t> order(raw.size());
+    for (size_t j = 0; j < order.size(); ++j) order[j] = (int64_t)j;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return raw[x].key < raw[y].key; });
+    std::vector<int32_t> id(raw.size());
+    int64_t m = 0;
+    for (size_t j = 0; j < order.size(); ++j) {
+        if (j == 0 || raw[order[j - 1]].key != raw[order[j]].key) {
+            std::memcpy(verts + 3 * m, raw[order[j]].p, sizeof(float) * 3);
+            ++m;
+        }
+        id[order[j]] = (int32_t)(m - 1);
+    }
+    int64_t t_out = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t packed = cubvh_amd::mc::kTriangles[cases[i]];
+        // the slot of edge e among the cell's crossing edges, in edge order
+        int slot[12], k = 0;
+        for (int e = 0; e < 12; ++e) {
+            const int a = kMcEdge[e][0], b = kMcEdge[e][1];
+            slot[e] = (((cases[i] >> a) ^ (cases[i] >> b)) & 1) ? k++ : -1;
+        }
+        for (int t = 0; t < cubvh_amd::mc::triangle_count(packed); ++t) {
+            const int e0 = cubvh_amd::mc::triangle_edge(packed, 3 * t), e1 = cubvh_amd::mc::triangle_edge(packed, 3 * t + 1),
+                      e2 = cubvh_amd::mc::triangle_edge(packed, 3 * t + 2);
+            if (slot[e0] < 0 || slot[e1] < 0 || slot[e2] < 0) return -1;  // table edge without a crossing
+            tris[3 * t_out] = id[first[i] + slot[e0]];
+            tris[3 * t_out + 1] = id[first[i] + slot[e2]];
+            tris[3 * t_out + 2] = id[first[i] + slot[e1]];
+            ++t_out;
+        }
+    }
+    *n_verts = m;
+    *n_tris = t_out;
+    return 0;
 }
 
 }  // extern "C"

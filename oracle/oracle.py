@@ -20,6 +20,7 @@ _lib = None
 
 REFERENCE = Path("/root/reference")
 REF_HASH_LIB = HERE / "_ref" / "libref_hashtable.so"
+REF_SMC_LIB = HERE / "_ref" / "libref_smc.so"
 
 
 def build(verbose: bool = False) -> None:
@@ -30,11 +31,11 @@ def build(verbose: bool = False) -> None:
         raise RuntimeError("oracle build failed:\n" + r.stdout)
     if verbose:
         print(r.stdout)
-    if (REFERENCE / "include" / "cpu" / "hashtable.h").exists():
+    if (REFERENCE / "include" / "cpu" / "hashtable.h").exists() and (REFERENCE / "include" / "cpu" / "spmc.h").exists():
         r = subprocess.run(["make", "-B", "-C", str(HERE), "ref", f"REF={REFERENCE}"], stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, text=True)
         if r.returncode != 0:
-            raise RuntimeError("reference hash table build failed:\n" + r.stdout)
+            raise RuntimeError("reference CPU hash table / marching cubes build failed:\n" + r.stdout)
 
 
 def _cpu_has_fma() -> bool:
@@ -85,6 +86,7 @@ def load():
         "orc_floodfill_bfs": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
         "orc_hash_build": (i64, [vp, i64, c_int, c_int, vp]),
         "orc_hash_search": (None, [vp, c_int, vp, c_int, vp, i64, vp]),
+        "orc_smc": (c_int, [vp, vp, i64, c_float, c_int, vp, vp, vp, vp]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -368,3 +370,65 @@ def ref_hash_build_search(coords, queries, capacity=None):
                                           _p(table))
     assert got == cap
     return out, table
+
+
+# ---------------------------------------------------------------- sparse marching cubes
+def smc(coords, corners, iso, consistent=False):
+    """The reference GPU extraction's output, restated: (vertices f32 [M, 3], triangles
+    int32 [T, 3])."""
+    c = _i32(coords).reshape(-1, 3)
+    v = _f32(corners).reshape(-1, 8)
+    n = c.shape[0]
+    verts = np.empty((12 * n + 1, 3), np.float32)
+    tris = np.empty((5 * n + 1, 3), np.int32)
+    nv = c_int64()
+    nt = c_int64()
+    rc = load().orc_smc(_p(c), _p(v), n, float(iso), int(bool(consistent)), _p(verts), _p(tris), ctypes.byref(nv),
+                        ctypes.byref(nt))
+    if rc != 0:
+        raise RuntimeError("case table uses an edge without a sign change")
+    return verts[: nv.value].copy(), tris[: nt.value].copy()
+
+
+_ref_smc = None
+
+
+def ref_smc_available() -> bool:
+    return REF_SMC_LIB.exists()
+
+
+def _ref_smc_lib():
+    global _ref_smc
+    if _ref_smc is None:
+        lib = ctypes.CDLL(str(REF_SMC_LIB))
+        lib.ref_mc_tables.restype = None
+        lib.ref_mc_tables.argtypes = [c_void_p, c_void_p]
+        lib.ref_smc.restype = None
+        lib.ref_smc.argtypes = [c_void_p, c_void_p, c_int, c_float, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                c_void_p, c_void_p]
+        _ref_smc = lib
+    return _ref_smc
+
+
+def ref_mc_tables():
+    """The reference's case tables (include/cpu/spmc.h): edge masks int32 [256] and
+    triangle lists int32 [256, 16]."""
+    edges = np.empty(256, np.int32)
+    tris = np.empty((256, 16), np.int32)
+    _ref_smc_lib().ref_mc_tables(_p(edges), _p(tris))
+    return edges, tris
+
+
+def ref_smc_cpu(coords, corners, iso, consistent=False):
+    """The reference's own CPU extraction (include/cpu/spmc.h, built into _ref/):
+    (vertices in first-use order, triangles)."""
+    c = _i32(coords).reshape(-1, 3)
+    v = _f32(corners).reshape(-1, 8)
+    n = c.shape[0]
+    verts = np.empty((12 * n + 1, 3), np.float32)
+    tris = np.empty((5 * n + 1, 3), np.int32)
+    nv = c_int64()
+    nt = c_int64()
+    _ref_smc_lib().ref_smc(_p(c), _p(v), n, float(iso), int(bool(consistent)), _p(verts), len(verts), _p(tris),
+                           len(tris), ctypes.byref(nv), ctypes.byref(nt))
+    return verts[: nv.value].copy(), tris[: nt.value].copy()
