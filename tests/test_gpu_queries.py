@@ -192,6 +192,12 @@ def test_sdf_raystab_bit_exact_closed_and_soup(gpu, oracle, ico4):
     np.testing.assert_array_equal(_np(fi), of)
     np.testing.assert_array_equal(_np(uvw), ou)
     assert (od < 0).any() and (od > 0).any()
+    # an odd count: the refilled raystab grid's last chunk is partial
+    p3 = S.uniform_points(10037, seed=9)
+    d, fi, _ = bvh2.signed_distance(torch.from_numpy(p3).to(gpu), mode="raystab")
+    od, of, _ = oracle.sdf_brute(V2, F2, p3, 1, return_uvw=False)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
 
 
 def test_state_dict_roundtrip_on_gpu(gpu, oracle, ico4):
