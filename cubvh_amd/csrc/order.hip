@@ -32,10 +32,14 @@ __global__ void bounds_init(int32_t* b) {
 }
 
 // Bounds of every `stride`-th query (the ordering only needs approximate bounds:
-// queries outside them are clamped to the boundary cells).
+// queries outside them are clamped to the boundary cells).  Waves reduce with
+// shuffles, blocks through LDS, and one lane per block updates the six bounds: few
+// blocks, so that the same-address atomics stay few.
 __global__ __launch_bounds__(256) void bounds_kernel(int64_t n, int64_t stride, const float* __restrict__ p,
                                                      int32_t* b) {
+    __shared__ float sh[6][4];
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll 4
     for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * stride; i < n;
          i += (int64_t)gridDim.x * blockDim.x * stride) {
 #pragma unroll
@@ -52,10 +56,21 @@ __global__ __launch_bounds__(256) void bounds_kernel(int64_t n, int64_t stride, 
             hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off, 64));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) {
+            sh[k][w] = lo[k];
+            sh[3 + k][w] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            if (lo[k] <= hi[k]) {  // skips waves that saw no (finite) query
+            for (int j = 1; j < 4; ++j) {
+                lo[k] = fminf(lo[k], sh[k][j]);
+                hi[k] = fmaxf(hi[k], sh[3 + k][j]);
+            }
+            if (lo[k] <= hi[k]) {  // skips blocks that saw no (finite) query
                 atomicMin(b + k, ordered(lo[k]));
                 atomicMax(b + 3 + k, ordered(hi[k]));
             }
@@ -97,7 +112,7 @@ hipError_t launch_bounds(int64_t n, const float* positions, int32_t* bounds, hip
     if (n > 0) {
         const int64_t stride = std::max<int64_t>(1, n >> 20);
         const int64_t m = (n + stride - 1) / stride;
-        const unsigned nb = (unsigned)std::min<int64_t>((m + 255) / 256, 1024);
+        const unsigned nb = (unsigned)std::min<int64_t>((m + 1023) / 1024, 256);  // >= 4 samples per thread
         hipLaunchKernelGGL(dev::bounds_kernel, dim3(nb), dim3(256), 0, s, n, stride, positions, bounds);
     }
     return hipGetLastError();
