@@ -112,6 +112,20 @@ def test_full_table_answers_stay_valid(capacity):
     assert (out[2000:] == -1).all()  # absent keys end after one full cycle
 
 
+def test_table_exactly_full():
+    """capacity == rows: every row is stored (no empty slot is left), present keys are
+    found and absent ones end after one full cycle."""
+    rng = np.random.default_rng(21)
+    coords = np.unique(rng.integers(-50, 50, size=(3000, 3)), axis=0).astype(np.int32)
+    coords = coords[rng.permutation(len(coords))]
+    impl = cubvh_amd.create_cuHashTable()
+    impl.resize(len(coords))
+    impl.insert(_t(coords))
+    np.testing.assert_array_equal(impl.search(_t(coords)).cpu().numpy(), np.arange(len(coords), dtype=np.int32))
+    absent = rng.integers(60, 90, size=(500, 3), dtype=np.int32)
+    assert (impl.search(_t(absent)).cpu().numpy() == -1).all()
+
+
 def test_sparse_voxel_shell_at_scale():
     """2^22-row sparse voxel set, 2^23 queries: every answer equals the brute force."""
     res = 1024

@@ -56,6 +56,14 @@ def test_wide_lattice_span_equals_oracle(consistent):
     _check(coords, corners, 0.0, consistent)
 
 
+@pytest.mark.parametrize("span", [1022, 1023])
+def test_key_width_boundary(span):
+    """A span of 1022 lattice steps still fits 32-bit edge keys, 1023 needs 64-bit."""
+    coords, corners = cells_of(sphere_field(12))
+    far = coords + np.array([span - 11, 3, 5], np.int32)
+    _check(np.concatenate([coords, far]), np.concatenate([corners, corners]), 0.0)
+
+
 def test_noisy_shared_corners_averaged_equal_oracle():
     coords, corners = cells_of(sphere_field(40))
     corners = corners + np.random.default_rng(2).normal(scale=0.05, size=corners.shape).astype(np.float32)
