@@ -290,14 +290,17 @@ __device__ int32_t closest_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N
     return bslot;
 }
 
-// Any hit along both rays o + t d and o - t d (t in [0, MAX_DIST)): bit 0 for +d, bit 1
-// for -d; the same answers as two separate any-hit walks, the reference's stab pair
-// (src/bvh.cu:414-424: a face hit with t < MAX_DIST along each).  One slab test gives
+// Whether both rays o + t d and o - t d (t in [0, MAX_DIST)) hit a face: returns 3 when
+// both do (the reference's stab pair, src/bvh.cu:414-424: a face hit with t < MAX_DIST
+// along each), otherwise a value other than 3 -- the walk stops as soon as one
+// direction is answered "no hit", since the pair is then decided.  One slab test gives
 // the line's interval [tn, tf] in a box: the +d ray meets it when tf >= 0 and
 // tn <= limit, the -d ray when tn <= 0 and tf >= -limit (the two rays' own slab tests,
 // negated exactly).  Children are walked nearest-first by the entry distance of the
 // directions still open; a stack entry carries those directions in the two low bits of
-// its key and is dropped at pop when both are answered.
+// its key and is dropped at pop when both are answered.  open1 / open2 count the stack
+// entries still open for +d / -d: a direction not yet hit with no open entry and not
+// open in the subtree at hand can no longer be hit.
 template <int BLOCK, int LDS_N>
 __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
     const f3 inv = ray_inverse(d);
@@ -305,6 +308,7 @@ __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& 
     int found = 0;
     int32_t ref = 0;
     int dirs = 3;
+    int open1 = 0, open2 = 0;
     st.sp = 0;
     for (;;) {
         if ((dirs & ~found) != 0) {
@@ -330,11 +334,18 @@ __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& 
                 }
                 sort4(k, r);
                 if (k[0] != INFINITY) {
-                    if (k[3] != INFINITY) st.push(r[3], k[3]);
-                    if (k[2] != INFINITY) st.push(r[2], k[2]);
-                    if (k[1] != INFINITY) st.push(r[1], k[1]);
+#pragma unroll
+                    for (int j = 3; j >= 1; --j)
+                        if (k[j] != INFINITY) {
+                            st.push(r[j], k[j]);
+                            const uint32_t m = __float_as_uint(k[j]) & 3u;
+                            open1 += (int)(m & 1u);
+                            open2 += (int)(m >> 1);
+                        }
                     ref = r[0];
                     dirs = (int)(__float_as_uint(k[0]) & 3u);
+                    const int pending = dirs | (open1 ? 1 : 0) | (open2 ? 2 : 0);
+                    if ((3 & ~found & ~pending) != 0) return found;  // a direction without a hit left
                     continue;
                 }
             } else {
@@ -349,10 +360,13 @@ __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& 
                 if (found == 3) return 3;
             }
         }
+        if ((3 & ~found & ~((open1 ? 1 : 0) | (open2 ? 2 : 0))) != 0) return found;
         if (st.sp == 0) return found;
         --st.sp;
         ref = st.ref_at(st.sp);
         dirs = (int)(__float_as_uint(st.key_at(st.sp)) & 3u);
+        open1 -= dirs & 1;
+        open2 -= dirs >> 1;
     }
 }
 
@@ -851,7 +865,7 @@ __global__ __launch_bounds__(BLOCK) void stab_sign_kernel(int64_t n, const float
     float oy = rng.next_float();
     for (uint32_t k = 0; k < 32; ++k) {
         f3 dir = fibonacci_dir32(k, ox, oy);
-        if (any_hit_line(T, p, dir, st) != 3) return;  // a miss along +dir or -dir: outside
+        if (any_hit_line(T, p, dir, st) != 3) return;  // no hit along +dir or -dir: outside
     }
     distances[q] = -distances[q];
 }
