@@ -4,7 +4,8 @@ brute-force dictionary.
 The answer for a key is the lowest row index stored with it, or -1: bit-exact with
 the oracle whenever the table has room for every row.  A full table drops rows the
 reference's GPU table also drops in an unspecified order; there the answers are
-checked to be valid instead.  Scenarios follow the reference's test/hashtable.py.
+checked to be valid instead.  Scenarios follow the reference's test/cuhashtable.py (the GPU
+form of test/hashtable.py).
 """
 import numpy as np
 import pytest
