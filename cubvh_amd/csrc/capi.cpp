@@ -435,7 +435,10 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "mesh must have vertices and faces");
     if (n_faces > MAX_LEAF_FIRST || n_vertices > (int64_t)0x7fffffff)
         return fail(CUBVH_ERR_ARG, "too many faces or vertices for the GPU build");
-    if (leaf_size < 0 || leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..8");
+    if (leaf_size < 0 || (leaf_size & ~0x1ff)) return fail(CUBVH_ERR_ARG, "unknown bits in leaf_size");
+    const bool median = (leaf_size & 0x100) == 0;  // bit 8: Morton radix tree instead
+    leaf_size &= 0xff;
+    if (leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..8");
     if (leaf_size == 0) leaf_size = kDefaultLeaf;
     if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
     hipStream_t hs = (hipStream_t)stream;
@@ -466,7 +469,7 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     } guard{ws, hs};
     const GpuBuildWorkspace w = carve_gpu_build_workspace(ws, n_faces);
     GpuBuildResult res;
-    e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, w, res, hs);
+    e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, median, w, res, hs);
     if (e != hipSuccess) return fail_hip(e, "GPU tree build");
     if (res.input_errors & 1) return fail(CUBVH_ERR_ARG, "a face references a vertex outside [0, n_vertices)");
     if (res.input_errors & 2) return fail(CUBVH_ERR_ARG, "vertex coordinates must be finite");

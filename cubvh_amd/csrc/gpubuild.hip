@@ -201,6 +201,116 @@ __global__ __launch_bounds__(256) void radix_tree(int64_t n, const uint64_t* __r
     if (i == 0) b.parent = -1;
 }
 
+// ---- median-split topology
+// The tree of a median build depends only on n and the leaf size: every run of more
+// than leaf_size slots [b, e) is halved at b + (e - b) / 2.  The faces are placed so that
+// the lower half holds the faces whose centroids come first, by (centroid, face index),
+// on the run's axis of largest centroid extent: the host build's object-median split
+// (build.cpp), done level by level over three centroid-sorted face lists.
+
+// Run [b, e) that slot p lies in after `level` halvings (runs of <= leaf slots stop).
+__device__ __forceinline__ void run_at(int64_t p, int64_t n, int level, int32_t leaf, int64_t& b, int64_t& e) {
+    b = 0;
+    e = n;
+    for (int l = 0; l < level && e - b > leaf; ++l) {
+        const int64_t m = b + (e - b) / 2;
+        if (p < m)
+            e = m;
+        else
+            b = m;
+    }
+}
+
+// Centroids, axis-major (cen[a * n + f]), as the gather computes them, and the sort
+// keys of axis 0: the centroid's bits made order-preserving (-0 counts as +0).
+__device__ __forceinline__ uint32_t float_key(float x) {
+    if (x == 0.0f) x = 0.0f;
+    const uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__global__ __launch_bounds__(256) void centroids(int64_t n, const float* __restrict__ tri9, float* __restrict__ cen) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    for (int a = 0; a < 3; ++a)
+        cen[a * n + f] = (tri9[9 * f + a] + tri9[9 * f + 3 + a] + tri9[9 * f + 6 + a]) * (1.0f / 3.0f);
+}
+__global__ __launch_bounds__(256) void axis_keys(int64_t n, const float* __restrict__ c, uint32_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    keys[f] = float_key(c[f]);
+    vals[f] = (uint32_t)f;
+}
+
+// Axis of largest centroid extent of run [b, e) (the first on ties, as the host).
+__device__ __forceinline__ int run_axis(const uint32_t* const* L, const float* __restrict__ cen, int64_t n, int64_t b,
+                                        int64_t e) {
+    int axis = 0;
+    float ext = cen[L[0][e - 1]] - cen[L[0][b]];
+    for (int a = 1; a < 3; ++a) {
+        const float x = cen[a * n + L[a][e - 1]] - cen[a * n + L[a][b]];
+        if (x > ext) {
+            ext = x;
+            axis = a;
+        }
+    }
+    return axis;
+}
+
+struct Lists {
+    const uint32_t* l[3];
+};
+
+// side[f] = 0 for the faces of the lower half of each run still being split, 1 above.
+__global__ __launch_bounds__(256) void median_mark(int64_t n, int level, int32_t leaf, Lists L,
+                                                   const float* __restrict__ cen, uint8_t* __restrict__ side) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int64_t b, e;
+    run_at(p, n, level, leaf, b, e);
+    if (e - b <= leaf) return;
+    const int axis = run_axis(L.l, cen, n, b, e);
+    side[L.l[axis][p]] = p < b + (e - b) / 2 ? 0 : 1;
+}
+
+// Lower-half flags of the three lists at slot p, scanned together.
+struct U3 {
+    uint32_t x, y, z;
+};
+struct U3Sum {
+    __host__ __device__ U3 operator()(const U3& a, const U3& b) const { return U3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+};
+struct LowerFlags {
+    Lists L;
+    const uint8_t* side;
+    __host__ __device__ U3 operator()(int64_t p) const {
+        return U3{side[L.l[0][p]] == 0 ? 1u : 0u, side[L.l[1][p]] == 0 ? 1u : 0u, side[L.l[2][p]] == 0 ? 1u : 0u};
+    }
+};
+
+// Stable partition of each run of each list by side (S: exclusive scan of the flags);
+// runs no longer split are copied.
+__global__ __launch_bounds__(256) void median_scatter(int64_t n, int level, int32_t leaf, Lists L, uint32_t* N0,
+                                                      uint32_t* N1, uint32_t* N2, const U3* __restrict__ S,
+                                                      const uint8_t* __restrict__ side) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int64_t b, e;
+    run_at(p, n, level, leaf, b, e);
+    uint32_t* out[3] = {N0, N1, N2};
+    if (e - b <= leaf) {
+        for (int a = 0; a < 3; ++a) out[a][p] = L.l[a][p];
+        return;
+    }
+    const int64_t m = b + (e - b) / 2;
+    const U3 sp = S[p], sb = S[b];
+    const uint32_t below[3] = {sp.x - sb.x, sp.y - sb.y, sp.z - sb.z};
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t f = L.l[a][p];
+        out[a][side[f] == 0 ? b + below[a] : m + (p - b - below[a])] = f;
+    }
+}
+
 // Outward rounding by a relative and an absolute margin (the host build's rule).
 __device__ __forceinline__ float pad_down(float v, double abs_pad) {
     return __double2float_rd((double)v - fabs((double)v) * 1.0e-6 - abs_pad);
@@ -311,6 +421,91 @@ __device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int3
         }
     }
     wn[ent.y] = w;
+}
+
+// Median topology: one wide node per thread, entry (first slot, end slot, wide node,
+// depth).  Its children are the run's halves, each halved once more when it is still
+// longer than leaf_size; runs of at most leaf_size slots become leaves.
+__device__ __forceinline__ void collapse_run_entry(const int4* __restrict__ in, int32_t e, int32_t n_in,
+                                                   int4* __restrict__ out, int32_t* n_out, int32_t leaf_size,
+                                                   WideNode* __restrict__ wn, int4* __restrict__ leaves,
+                                                   int4* __restrict__ jobs, int32_t* c) {
+    const bool valid = e < n_in;
+    int4 ent = make_int4(0, 0, 0, 0);
+    int32_t kb[4] = {0, 0, 0, 0}, ke[4] = {0, 0, 0, 0};
+    int nk = 0;
+    if (valid) {
+        ent = in[e];
+        const int32_t b = ent.x, end = ent.y, m = b + (end - b) / 2;
+        const int32_t hb[2] = {b, m}, he[2] = {m, end};
+        for (int h = 0; h < 2; ++h) {
+            if (he[h] - hb[h] > leaf_size) {
+                const int32_t q = hb[h] + (he[h] - hb[h]) / 2;
+                kb[nk] = hb[h];
+                ke[nk++] = q;
+                kb[nk] = q;
+                ke[nk++] = he[h];
+            } else {
+                kb[nk] = hb[h];
+                ke[nk++] = he[h];
+            }
+        }
+    }
+    int32_t n_leaf = 0, n_int = 0;
+    for (int q = 0; q < nk; ++q) {
+        if (ke[q] - kb[q] <= leaf_size) ++n_leaf;
+        else ++n_int;
+    }
+    int32_t tot_leaf, tot_int;
+    const int32_t off_leaf = wave_scan(n_leaf, tot_leaf);
+    const int32_t off_int = wave_scan(n_int, tot_int);
+    int32_t b_leaf = 0, b_wide = 0, b_queue = 0, b_job = 0;
+    if ((threadIdx.x & 63) == 0) {
+        if (tot_leaf) b_leaf = atomicAdd(&c[10], tot_leaf);
+        if (tot_int) {
+            b_wide = atomicAdd(&c[8], tot_int);
+            b_queue = atomicAdd(n_out, tot_int);
+            b_job = atomicAdd(&c[11], tot_int);
+        }
+    }
+    b_leaf = __shfl(b_leaf, 0) + off_leaf;
+    b_wide = __shfl(b_wide, 0) + off_int;
+    b_queue = __shfl(b_queue, 0) + off_int;
+    b_job = __shfl(b_job, 0) + off_int;
+    int32_t depth = valid ? ent.z : 0;
+    for (int o = 32; o > 0; o >>= 1) depth = max(depth, __shfl_xor(depth, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&c[12], depth);
+    if (!valid) return;
+    WideNode w;
+    for (int q = 0; q < 4; ++q) {
+        w.lo_x[q] = w.lo_y[q] = w.lo_z[q] = INFINITY;
+        w.hi_x[q] = w.hi_y[q] = w.hi_z[q] = -INFINITY;
+        w.child[q] = CHILD_EMPTY;
+    }
+    w.n_children = nk;
+    w.pad[0] = w.pad[1] = w.pad[2] = 0;
+    for (int q = 0; q < nk; ++q) {
+        const int32_t first = kb[q], cnt = ke[q] - kb[q];
+        if (cnt <= leaf_size) {
+            w.child[q] = (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(cnt - 1));  // leaf reference
+            leaves[b_leaf++] = make_int4(first, cnt, ent.w, q);
+        } else {
+            const int32_t id = b_wide++;
+            w.child[q] = id;
+            out[b_queue++] = make_int4(first, first + cnt, ent.z + 1, id);
+            jobs[b_job++] = make_int4(ent.w, q, first, cnt);
+        }
+    }
+    wn[ent.w] = w;
+}
+
+__global__ __launch_bounds__(256) void collapse_run_level(const int4* __restrict__ in, const int32_t* n_in_ptr,
+                                                          int4* __restrict__ out, int32_t* n_out, int32_t leaf_size,
+                                                          WideNode* __restrict__ wn, int4* __restrict__ leaves,
+                                                          int4* __restrict__ jobs, int32_t* c) {
+    const int32_t n_in = n_in_ptr ? *n_in_ptr : 1;  // the first level holds the root alone
+    for (int32_t e0 = (int32_t)(blockIdx.x * blockDim.x); e0 < n_in; e0 += (int32_t)(gridDim.x * blockDim.x))
+        collapse_run_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, leaf_size, wn, leaves, jobs, c);
 }
 
 __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, const int32_t* n_in_ptr,
@@ -618,40 +813,72 @@ __global__ __launch_bounds__(BLOCK) void subtree_records(const int4* __restrict_
 
 }  // namespace gb
 
-size_t gpu_build_workspace_bytes(int64_t n) {
-    size_t sort_tmp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+namespace {
+
+size_t sort_tmp_bytes(int64_t n) {
+    size_t a = 0, b = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, a, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63);
+    hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
+    return std::max(a, b);
+}
+
+using FlagIter = hipcub::TransformInputIterator<gb::U3, gb::LowerFlags, hipcub::CountingInputIterator<int64_t>>;
+
+size_t flag_scan_bytes(int64_t n) {
+    size_t b = 0;
+    FlagIter it(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{});
+    hipcub::DeviceScan::ExclusiveScan(nullptr, b, it, (gb::U3*)nullptr, gb::U3Sum(), gb::U3{0, 0, 0}, (int)n);
+    return b;
+}
+
+// The workspace layout, measured (ws == nullptr) or carved.
+size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const int64_t ni = n > 1 ? n - 1 : 1;
-    return al((64 + 8 * gb::kGatherBlocks) * sizeof(int32_t)) + al(9 * n * sizeof(float)) + 2 * al(n * sizeof(uint64_t)) +
-           2 * al(n * sizeof(uint32_t)) + al(sort_tmp) + al(ni * sizeof(BinNode)) + al(n * sizeof(int32_t)) +
-           al(ni * sizeof(WideNode)) + 2 * al(ni * sizeof(int4)) + al(n * sizeof(int4)) + al(ni * sizeof(int4));
+    char* base = (char*)ws;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* q = base ? base + off : nullptr;
+        off += al(bytes);
+        return q;
+    };
+    w.counters = (int32_t*)take((64 + 8 * gb::kGatherBlocks) * sizeof(int32_t));
+    w.tri9 = (float*)take(9 * n * sizeof(float));
+    w.keys0 = (uint64_t*)take(n * sizeof(uint64_t));
+    w.keys1 = (uint64_t*)take(n * sizeof(uint64_t));
+    w.vals0 = (uint32_t*)take(n * sizeof(uint32_t));
+    w.face_of = (uint32_t*)take(n * sizeof(uint32_t));
+    w.sort_tmp_bytes = sort_tmp_bytes(n);
+    w.sort_tmp = take(w.sort_tmp_bytes);
+    w.bn = (BinNode*)take(ni * sizeof(BinNode));
+    w.leaf_parent = (int32_t*)take(n * sizeof(int32_t));
+    w.wide = (WideNode*)take(ni * sizeof(WideNode));
+    w.queue0 = (int4*)take(ni * sizeof(int4));
+    w.queue1 = (int4*)take(ni * sizeof(int4));
+    w.leaves = (int4*)take(n * sizeof(int4));
+    w.jobs = (int4*)take(ni * sizeof(int4));
+    w.cen = (float*)take(3 * n * sizeof(float));
+    w.lists = (uint32_t*)take(6 * n * sizeof(uint32_t));
+    w.side = (uint8_t*)take(n);
+    w.flag_scan = take(n * sizeof(gb::U3));
+    w.flag_tmp_bytes = flag_scan_bytes(n);
+    w.flag_tmp = take(w.flag_tmp_bytes);
+    w.n = n;
+    return off;
+}
+
+}  // namespace
+
+size_t gpu_build_workspace_bytes(int64_t n) {
+    GpuBuildWorkspace w;
+    return lay_out_workspace(nullptr, n, w);
 }
 
 GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
-    size_t sort_tmp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63);
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const int64_t ni = n > 1 ? n - 1 : 1;
-    char* p = (char*)ws;
     GpuBuildWorkspace w;
-    w.counters = (int32_t*)p; p += al((64 + 8 * gb::kGatherBlocks) * sizeof(int32_t));
-    w.tri9 = (float*)p; p += al(9 * n * sizeof(float));
-    w.keys0 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
-    w.keys1 = (uint64_t*)p; p += al(n * sizeof(uint64_t));
-    w.vals0 = (uint32_t*)p; p += al(n * sizeof(uint32_t));
-    w.face_of = (uint32_t*)p; p += al(n * sizeof(uint32_t));
-    w.sort_tmp = p; w.sort_tmp_bytes = sort_tmp; p += al(sort_tmp);
-    w.bn = (BinNode*)p; p += al(ni * sizeof(BinNode));
-    w.leaf_parent = (int32_t*)p; p += al(n * sizeof(int32_t));
-    w.wide = (WideNode*)p; p += al(ni * sizeof(WideNode));
-    w.queue0 = (int4*)p; p += al(ni * sizeof(int4));
-    w.queue1 = (int4*)p; p += al(ni * sizeof(int4));
-    w.leaves = (int4*)p; p += al(n * sizeof(int4));
-    w.jobs = (int4*)p; p += al(ni * sizeof(int4));
-    w.n = n;
+    lay_out_workspace(ws, n, w);
     return w;
 }
 
@@ -660,26 +887,69 @@ unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_
 constexpr int kMaxDepth = 21;  // 3 * 21 + 1 = 64 stack entries
 }  // namespace
 
-hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
+// Median topology, steps 2-3: the three centroid-sorted face lists, then one stable
+// partition of every run per level until all runs are leaves; the first list is the
+// leaf order.
+hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, hipStream_t s) {
+    hipLaunchKernelGGL(gb::centroids, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.cen);
+    uint32_t* cur[3] = {w.lists, w.lists + n, w.lists + 2 * n};
+    uint32_t* nxt[3] = {w.lists + 3 * n, w.lists + 4 * n, w.lists + 5 * n};
+    uint32_t* keys = reinterpret_cast<uint32_t*>(w.keys0);
+    uint32_t* keys_sorted = reinterpret_cast<uint32_t*>(w.keys1);
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(gb::axis_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.cen + a * n, keys, w.vals0);
+        size_t tmp = w.sort_tmp_bytes;
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp, keys, keys_sorted, w.vals0, cur[a], (int)n,
+                                                          0, 32, s);
+        if (e != hipSuccess) return e;
+    }
+    int levels = 0;
+    for (int64_t run = n; run > leaf_size; run -= run / 2) ++levels;
+    gb::U3* S = reinterpret_cast<gb::U3*>(w.flag_scan);
+    for (int level = 0; level < levels; ++level) {
+        const gb::Lists L{{cur[0], cur[1], cur[2]}};
+        hipLaunchKernelGGL(gb::median_mark, dim3(blocks_for(n)), dim3(256), 0, s, n, level, leaf_size, L, w.cen,
+                           w.side);
+        FlagIter flags(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{L, w.side});
+        size_t tmp = w.flag_tmp_bytes;
+        hipError_t e = hipcub::DeviceScan::ExclusiveScan(w.flag_tmp, tmp, flags, S, gb::U3Sum(), gb::U3{0, 0, 0},
+                                                         (int)n, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(gb::median_scatter, dim3(blocks_for(n)), dim3(256), 0, s, n, level, leaf_size, L, nxt[0],
+                           nxt[1], nxt[2], S, w.side);
+        for (int a = 0; a < 3; ++a) std::swap(cur[a], nxt[a]);
+    }
+    return hipMemcpyAsync(w.face_of, cur[0], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, bool median,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s) {
     hipError_t e = hipSuccess;
     hipLaunchKernelGGL(gb::init_counters, dim3(1), dim3(64), 0, s, w.counters);
     const unsigned gblocks = std::min<unsigned>(blocks_for(n), gb::kGatherBlocks);
     hipLaunchKernelGGL(gb::gather_faces, dim3(gblocks), dim3(256), 0, s, n, V, nv, F, w.tri9, w.counters + 64);
     hipLaunchKernelGGL(gb::fold_partials, dim3(1), dim3(256), 0, s, w.counters + 64, (int32_t)gblocks, w.counters);
-    hipLaunchKernelGGL(gb::morton_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.counters, w.keys0,
-                       w.vals0);
-    size_t tmp = w.sort_tmp_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp, w.keys0, w.keys1, w.vals0, w.face_of, (int)n, 0, 63, s);
-    if (e != hipSuccess) return e;
-    const int64_t ni = n - 1;
-    hipLaunchKernelGGL(gb::radix_tree, dim3(blocks_for(ni)), dim3(256), 0, s, n, w.keys1, w.bn, w.leaf_parent);
+    if (median) {
+        e = median_order(n, leaf_size, w, s);
+        if (e != hipSuccess) return e;
+    } else {
+        hipLaunchKernelGGL(gb::morton_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.counters, w.keys0,
+                           w.vals0);
+        size_t tmp = w.sort_tmp_bytes;
+        e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp, w.keys0, w.keys1, w.vals0, w.face_of, (int)n, 0, 63,
+                                               s);
+        if (e != hipSuccess) return e;
+        const int64_t ni = n - 1;
+        hipLaunchKernelGGL(gb::radix_tree, dim3(blocks_for(ni)), dim3(256), 0, s, n, w.keys1, w.bn, w.leaf_parent);
+    }
     // collapse: one launch per level for the deepest tree a walk accepts (depth
     // kMaxDepth, 3 * depth + 1 <= 64 stack entries), without host synchronization: level
     // k reads its entry count from counters[16 + k - 1] (the root alone at level 0) and
     // counts its outputs in counters[16 + k]; a tree still growing after the last
-    // level is reported as too deep.  The root entry is (binary root 0, wide node 0, 1).
-    const int4 root = make_int4(0, 0, 1, 0);
+    // level is reported as too deep.  The root entry is (binary root 0, wide node 0,
+    // depth 1) for the radix tree, (slots 0 .. n, depth 1, wide node 0) for the median
+    // topology.
+    const int4 root = median ? make_int4(0, (int32_t)n, 1, 0) : make_int4(0, 0, 1, 0);
     const int32_t one = 1;
     e = hipMemcpyAsync(w.queue0, &root, sizeof root, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(w.counters + 8, &one, sizeof one, hipMemcpyHostToDevice, s);
@@ -688,9 +958,13 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     for (int level = 0; level < kMaxDepth; ++level) {
         const int4* qin = (level & 1) ? w.queue1 : w.queue0;
         int4* qout = (level & 1) ? w.queue0 : w.queue1;
-        hipLaunchKernelGGL(gb::collapse_level, dim3(cblocks), dim3(256), 0, s, qin,
-                           level == 0 ? (const int32_t*)nullptr : w.counters + 16 + level - 1, qout,
-                           w.counters + 16 + level, w.bn, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+        const int32_t* n_in = level == 0 ? (const int32_t*)nullptr : w.counters + 16 + level - 1;
+        if (median)
+            hipLaunchKernelGGL(gb::collapse_run_level, dim3(cblocks), dim3(256), 0, s, qin, n_in, qout,
+                               w.counters + 16 + level, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+        else
+            hipLaunchKernelGGL(gb::collapse_level, dim3(cblocks), dim3(256), 0, s, qin, n_in, qout,
+                               w.counters + 16 + level, w.bn, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
     }
     // every sorted face in exactly one leaf (leaf_parent is free again: reused as counts)
     hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, w.leaf_parent, n,

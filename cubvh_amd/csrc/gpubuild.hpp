@@ -36,6 +36,13 @@ struct GpuBuildWorkspace {
     int4* queue1 = nullptr;
     int4* leaves = nullptr;       // (first, count, wide node, slot) per leaf child
     int4* jobs = nullptr;         // (wide node, slot, first, count) per internal child
+    // median topology
+    float* cen = nullptr;         // [3n] centroids, axis-major
+    uint32_t* lists = nullptr;    // [6n] three centroid-sorted face lists, double-buffered
+    uint8_t* side = nullptr;      // [n] lower (0) or upper (1) half of the face's run
+    void* flag_scan = nullptr;    // [n] scanned lower-half flags of the three lists
+    void* flag_tmp = nullptr;
+    size_t flag_tmp_bytes = 0;
     int64_t n = 0;
 };
 
@@ -49,11 +56,14 @@ struct GpuBuildResult {
 size_t gpu_build_workspace_bytes(int64_t n);
 GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n);
 
-// Steps 1-4 (n >= 2 faces): wide nodes (without boxes) in w.wide, leaf order in
-// w.face_of.  res.input_errors is set when an index or coordinate is bad (the tree is
-// then meaningless, but every access stayed in bounds); res.too_deep when the tree is
-// deeper than a walk's stack accepts.  Synchronizes the stream once.
-hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size,
+// Steps 1-4 (n > leaf_size faces): wide nodes (without boxes) in w.wide, leaf order in
+// w.face_of.  median: every run of faces is halved at the median centroid on its axis
+// of largest centroid extent (the host's object-median split); otherwise the Morton
+// radix tree (faster to build, slower to query).  res.input_errors is set when an index
+// or coordinate is bad (the tree is then meaningless, but every access stayed in
+// bounds); res.too_deep when the tree is deeper than a walk's stack accepts.
+// Synchronizes the stream once.
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, bool median,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s);
 
 // Step 5: triangles and derived arrays into `out`, child boxes into w.wide (the caller

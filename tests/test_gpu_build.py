@@ -44,20 +44,46 @@ def _leaves_cover_faces(bvh, v, f):
     assert (seen == 1).all()
 
 
+@pytest.mark.parametrize("build", ["gpu", "gpu_lbvh"])
 @pytest.mark.parametrize("leaf", [0, 1, 4])
-def test_gpu_build_structure(gpu, leaf):
+def test_gpu_build_structure(gpu, leaf, build):
     v, f, _ = S.perturbed_sphere(5, seed=2)
-    bvh = cubvh_amd.cuBVH(v, f, leaf_size=leaf, build="gpu")
+    bvh = cubvh_amd.cuBVH(v, f, leaf_size=leaf, build=build)
     info = bvh.info()
     assert info["n_faces"] == len(f) and info["stack_need"] == 3 * info["depth"] + 1 <= 64
     assert info["leaf_size"] == (leaf or 8)
     _leaves_cover_faces(bvh, v, f)
 
 
-def test_gpu_build_queries_equal_host_tree_and_oracle(gpu, oracle, ico4):
+def _leaf_sets(bvh):
+    sd = bvh.state_dict()
+    nodes = sd["nodes"].numpy()
+    ids = sd["triangles"].numpy().view(np.int64)[:, 5]
+    out = set()
+    for r in nodes[:, 24:28].reshape(-1):
+        if r == -1 or r >= 0:
+            continue
+        u = np.uint32(r & 0x7FFFFFFF)
+        first, cnt = int(u >> 3), int(u & 7) + 1
+        out.add(frozenset(ids[first:first + cnt].tolist()))
+    return out
+
+
+@pytest.mark.parametrize("leaf", [0, 3])
+def test_gpu_median_build_has_the_host_median_leaves(gpu, leaf):
+    """The GPU median build halves every run at the median centroid of its longest axis,
+    the host's object-median rule: both trees have exactly the same leaves."""
+    v, f, _ = S.perturbed_sphere(6, seed=7)
+    g = cubvh_amd.cuBVH(v, f, leaf_size=leaf, build="gpu")
+    h = cubvh_amd.cuBVH(v, f, leaf_size=leaf | 0x100)  # host build, object median
+    assert _leaf_sets(g) == _leaf_sets(h)
+
+
+@pytest.mark.parametrize("build", ["gpu", "gpu_lbvh"])
+def test_gpu_build_queries_equal_host_tree_and_oracle(gpu, oracle, ico4, build):
     v, f = ico4
     p = torch.from_numpy(S.uniform_points(30000, seed=5)).to(gpu)
-    g = cubvh_amd.cuBVH(v, f, build="gpu")
+    g = cubvh_amd.cuBVH(v, f, build=build)
     h = cubvh_amd.cuBVH(v, f)
     for args in ((False,), (True,)):
         a = g.unsigned_distance(p, return_uvw=args[0])
