@@ -397,20 +397,21 @@ def run_extras(dev, steps, warmup, out):
     import ctypes
 
     stream = torch.cuda.current_stream(dev).cuda_stream
-    times = []
-    for _ in range(3):
-        h = ctypes.c_void_p()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        cubvh_amd._lib.check(lib.cubvh_bvh_create_gpu(dev.index, V8.data_ptr(), V8.shape[0], F8.data_ptr(),
-                                                      F8.shape[0], 0, ctypes.byref(h), stream))
-        times.append(time.perf_counter() - t0)
-        lib.cubvh_bvh_destroy(h)
-    gbvh = cubvh_amd.cuBVH(v8, f8, build="gpu")
-    _, t = _time_kernel(lambda: gbvh.unsigned_distance(G), steps, 1)
-    out["gpu_build_level8"] = {"build_ms": min(times) * 1e3, "udf_Mq_s": G.shape[0] / t / 1e6,
-                               "tree": gbvh.info()}
-    del gbvh, V8, F8
+    for name, flags, build in (("gpu_build_level8", 0, "gpu"), ("gpu_build_level8_lbvh", 0x100, "gpu_lbvh")):
+        times = []
+        for _ in range(3):
+            h = ctypes.c_void_p()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cubvh_amd._lib.check(lib.cubvh_bvh_create_gpu(dev.index, V8.data_ptr(), V8.shape[0], F8.data_ptr(),
+                                                          F8.shape[0], flags, ctypes.byref(h), stream))
+            times.append(time.perf_counter() - t0)
+            lib.cubvh_bvh_destroy(h)
+        gbvh = cubvh_amd.cuBVH(v8, f8, build=build)
+        _, t = _time_kernel(lambda: gbvh.unsigned_distance(G), steps, 1)
+        out[name] = {"build_ms": min(times) * 1e3, "udf_Mq_s": G.shape[0] / t / 1e6, "tree": gbvh.info()}
+        del gbvh
+    del V8, F8
 
     # floodfill of the occupancy udf < 2/256
     d, _, _ = bvh8.unsigned_distance(G)
