@@ -55,6 +55,12 @@ __global__ void init_counters(int32_t* c) {
 // (ordered ints; no atomics on shared counters); fold_partials combines them.
 constexpr int kGatherBlocks = 2048;
 
+// Internal children with more faces than this are reduced in kChunks chunks by separate
+// workgroups (their indices listed in bigjobs, counted in counters[9]); the rest by one
+// wave each.
+constexpr int32_t kWaveJobMax = 4096;
+constexpr int kChunks = 32;
+
 __global__ __launch_bounds__(256) void gather_faces(int64_t n, const float* __restrict__ V, int64_t nv,
                                                     const int32_t* __restrict__ F, float* __restrict__ tri9,
                                                     int32_t* __restrict__ part) {
@@ -311,6 +317,130 @@ __global__ __launch_bounds__(256) void median_scatter(int64_t n, int level, int3
     }
 }
 
+// The remaining halvings of one run of at most kFinishRun slots, in LDS: one workgroup
+// per run, the same marks, scans and stable scatters as the global levels, on local
+// face numbers; writes the run's final slot order into face_of.  loc: a scratch word
+// per face (local number).
+constexpr int kFinishRun = 2048;
+constexpr int kFinishThreads = 1024;
+
+__device__ __forceinline__ void local_run(int i, int k, int level, int32_t leaf, int& b, int& e) {
+    b = 0;
+    e = k;
+    for (int l = 0; l < level && e - b > leaf; ++l) {
+        const int m = b + (e - b) / 2;
+        if (i < m)
+            e = m;
+        else
+            b = m;
+    }
+}
+
+// Exclusive scan over the workgroup of one 64-bit value per element, two elements per
+// thread (elements 2t and 2t + 1); returns the scans of both.
+__device__ __forceinline__ void block_scan2(uint64_t x0, uint64_t x1, uint64_t& s0, uint64_t& s1, uint64_t* sh) {
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const uint64_t t = x0 + x1;
+    uint64_t v = t;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) sh[wave] = v;
+    __syncthreads();
+    uint64_t base = 0;
+    for (int w = 0; w < wave; ++w) base += sh[w];
+    s0 = base + v - t;
+    s1 = s0 + x0;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kFinishThreads) void median_finish(int64_t n, int global_levels, int32_t leaf, Lists L,
+                                                                const float* __restrict__ cen,
+                                                                uint32_t* __restrict__ loc,
+                                                                uint32_t* __restrict__ face_of) {
+    __shared__ uint32_t face[kFinishRun];
+    __shared__ float c[3][kFinishRun];
+    __shared__ uint16_t lst[2][3][kFinishRun];
+    __shared__ uint8_t side[kFinishRun];
+    __shared__ uint64_t sh[kFinishThreads / 64];
+    // this run: follow the bits of the block index down the halvings
+    int64_t b = 0, e = n;
+    for (int l = 0; l < global_levels; ++l) {
+        const int64_t m = b + (e - b) / 2;
+        if ((blockIdx.x >> (global_levels - 1 - l)) & 1u)
+            b = m;
+        else
+            e = m;
+    }
+    const int k = (int)(e - b);
+    for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) {
+        const uint32_t f = L.l[0][b + i];
+        face[i] = f;
+        loc[f] = (uint32_t)i;
+        for (int a = 0; a < 3; ++a) c[a][i] = cen[a * n + f];
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) {
+        lst[0][0][i] = (uint16_t)i;
+        lst[0][1][i] = (uint16_t)loc[L.l[1][b + i]];
+        lst[0][2][i] = (uint16_t)loc[L.l[2][b + i]];
+    }
+    __syncthreads();
+    int levels = 0;
+    for (int run = k; run > leaf; run -= run / 2) ++levels;
+    int cur = 0;
+    for (int level = 0; level < levels; ++level) {
+        // marks
+        for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) {
+            int rb, re;
+            local_run(i, k, level, leaf, rb, re);
+            if (re - rb <= leaf) continue;
+            int axis = 0;
+            float ext = c[0][lst[cur][0][re - 1]] - c[0][lst[cur][0][rb]];
+            for (int a = 1; a < 3; ++a) {
+                const float x = c[a][lst[cur][a][re - 1]] - c[a][lst[cur][a][rb]];
+                if (x > ext) {
+                    ext = x;
+                    axis = a;
+                }
+            }
+            side[lst[cur][axis][i]] = i < rb + (re - rb) / 2 ? 0 : 1;
+        }
+        __syncthreads();
+        // lower-half flags of the three lists, 16 bits each, scanned together
+        const int i0 = 2 * (int)threadIdx.x, i1 = i0 + 1;
+        uint64_t x0 = 0, x1 = 0;
+        for (int a = 0; a < 3; ++a) {
+            if (i0 < k && side[lst[cur][a][i0]] == 0) x0 |= 1ull << (16 * a);
+            if (i1 < k && side[lst[cur][a][i1]] == 0) x1 |= 1ull << (16 * a);
+        }
+        uint64_t s0, s1;
+        block_scan2(x0, x1, s0, s1, sh);
+        __shared__ uint64_t scan[kFinishRun];
+        if (i0 < k) scan[i0] = s0;
+        if (i1 < k) scan[i1] = s1;
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) {
+            int rb, re;
+            local_run(i, k, level, leaf, rb, re);
+            for (int a = 0; a < 3; ++a) {
+                const uint16_t j = lst[cur][a][i];
+                int pos = i;
+                if (re - rb > leaf) {
+                    const int rm = rb + (re - rb) / 2;
+                    const int below = (int)(((scan[i] - scan[rb]) >> (16 * a)) & 0xffffu);
+                    pos = side[j] == 0 ? rb + below : rm + (i - rb - below);
+                }
+                lst[cur ^ 1][a][pos] = j;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) face_of[b + i] = face[lst[cur][0][i]];
+}
+
 // Outward rounding by a relative and an absolute margin (the host build's rule).
 __device__ __forceinline__ float pad_down(float v, double abs_pad) {
     return __double2float_rd((double)v - fabs((double)v) * 1.0e-6 - abs_pad);
@@ -339,7 +469,7 @@ __device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int3
                                                int4* __restrict__ out, int32_t* n_out,
                                                const BinNode* __restrict__ bn, int32_t leaf_size,
                                                WideNode* __restrict__ wn, int4* __restrict__ leaves,
-                                               int4* __restrict__ jobs, int32_t* c) {
+                                               int4* __restrict__ jobs, int32_t* __restrict__ bigjobs, int32_t* c) {
     const bool valid = e < n_in;
     int4 ent = make_int4(0, 0, 0, 0);
     int32_t kids[4] = {0, 0, 0, 0};
@@ -417,6 +547,7 @@ __device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int3
             const int32_t id = b_wide++;
             w.child[q] = id;
             out[b_queue++] = make_int4(r, id, ent.z + 1, 0);
+            if (cnt > kWaveJobMax) bigjobs[atomicAdd(&c[9], 1)] = b_job;
             jobs[b_job++] = make_int4(ent.y, q, first, cnt);
         }
     }
@@ -429,7 +560,7 @@ __device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int3
 __device__ __forceinline__ void collapse_run_entry(const int4* __restrict__ in, int32_t e, int32_t n_in,
                                                    int4* __restrict__ out, int32_t* n_out, int32_t leaf_size,
                                                    WideNode* __restrict__ wn, int4* __restrict__ leaves,
-                                                   int4* __restrict__ jobs, int32_t* c) {
+                                                   int4* __restrict__ jobs, int32_t* __restrict__ bigjobs, int32_t* c) {
     const bool valid = e < n_in;
     int4 ent = make_int4(0, 0, 0, 0);
     int32_t kb[4] = {0, 0, 0, 0}, ke[4] = {0, 0, 0, 0};
@@ -493,6 +624,7 @@ __device__ __forceinline__ void collapse_run_entry(const int4* __restrict__ in, 
             const int32_t id = b_wide++;
             w.child[q] = id;
             out[b_queue++] = make_int4(first, first + cnt, ent.z + 1, id);
+            if (cnt > kWaveJobMax) bigjobs[atomicAdd(&c[9], 1)] = b_job;
             jobs[b_job++] = make_int4(ent.w, q, first, cnt);
         }
     }
@@ -502,20 +634,22 @@ __device__ __forceinline__ void collapse_run_entry(const int4* __restrict__ in, 
 __global__ __launch_bounds__(256) void collapse_run_level(const int4* __restrict__ in, const int32_t* n_in_ptr,
                                                           int4* __restrict__ out, int32_t* n_out, int32_t leaf_size,
                                                           WideNode* __restrict__ wn, int4* __restrict__ leaves,
-                                                          int4* __restrict__ jobs, int32_t* c) {
+                                                          int4* __restrict__ jobs, int32_t* __restrict__ bigjobs,
+                                                          int32_t* c) {
     const int32_t n_in = n_in_ptr ? *n_in_ptr : 1;  // the first level holds the root alone
     for (int32_t e0 = (int32_t)(blockIdx.x * blockDim.x); e0 < n_in; e0 += (int32_t)(gridDim.x * blockDim.x))
-        collapse_run_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, leaf_size, wn, leaves, jobs, c);
+        collapse_run_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, leaf_size, wn, leaves, jobs, bigjobs, c);
 }
 
 __global__ __launch_bounds__(256) void collapse_level(const int4* __restrict__ in, const int32_t* n_in_ptr,
                                                       int4* __restrict__ out, int32_t* n_out,
                                                       const BinNode* __restrict__ bn, int32_t leaf_size,
                                                       WideNode* __restrict__ wn, int4* __restrict__ leaves,
-                                                      int4* __restrict__ jobs, int32_t* c) {
+                                                      int4* __restrict__ jobs, int32_t* __restrict__ bigjobs,
+                                                      int32_t* c) {
     const int32_t n_in = n_in_ptr ? *n_in_ptr : 1;  // the first level holds the root alone
     for (int32_t e0 = (int32_t)(blockIdx.x * blockDim.x); e0 < n_in; e0 += (int32_t)(gridDim.x * blockDim.x))
-        collapse_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, bn, leaf_size, wn, leaves, jobs, c);
+        collapse_entry(in, e0 + (int32_t)threadIdx.x, n_in, out, n_out, bn, leaf_size, wn, leaves, jobs, bigjobs, c);
 }
 
 // Writes child slot q of wide node w: the box of its faces widened by the margin.
@@ -732,8 +866,6 @@ __device__ __forceinline__ void block_reduce(double* sh, double* v, int nv, int 
     __syncthreads();
 }
 
-// Internal children with more faces than this get a workgroup; the rest a wave.
-constexpr int32_t kWaveJobMax = 4096;
 
 __device__ __forceinline__ double wave_reduce(double v, int op) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -744,66 +876,47 @@ __device__ __forceinline__ double wave_reduce(double v, int op) {
 }
 
 // The box of an internal child's run of faces into its wide node and its disk at
-// cbnd[8 * wide + 2 * slot], job = (wide node, slot, first, count).  WAVE: one wave
-// per job of at most kWaveJobMax faces (shuffle reductions); else one workgroup per
-// larger job.
-template <bool WAVE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
-                                                         const float4* __restrict__ tris,
-                                                         float4* __restrict__ cbnd, WideNode* __restrict__ wn) {
-    __shared__ double sh[3 * (BLOCK / 64)];
+// cbnd[8 * wide + 2 * slot], job = (wide node, slot, first, count): one wave per job of
+// at most kWaveJobMax faces (shuffle reductions); larger jobs are the big_job_* passes'.
+__global__ __launch_bounds__(256) void subtree_records(const int4* __restrict__ jobs, const int32_t* c,
+                                                       const float4* __restrict__ tris, float4* __restrict__ cbnd,
+                                                       WideNode* __restrict__ wn) {
     const double abs_pad = margin_of(c);
-    const int lane = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-    const int width = WAVE ? 64 : BLOCK;
-    const int32_t per_block = WAVE ? BLOCK / 64 : 1;
-    const int32_t sub = WAVE ? (int32_t)(threadIdx.x >> 6) : 0;
-    for (int32_t e0 = blockIdx.x * per_block; e0 < c[11]; e0 += gridDim.x * per_block) {
-        const int32_t e = e0 + sub;
-        int4 job = make_int4(0, 0, 0, 0);
-        if (e < c[11]) job = jobs[e];
-        // each kernel takes only its own kind of job (uniform per wave / per block)
-        const bool mine = e < c[11] && (WAVE ? job.w <= kWaveJobMax : job.w > kWaveJobMax);
-        if (WAVE && !mine) continue;
-        if (!WAVE && !mine) continue;  // uniform over the block: one job per block
+    const int lane = (int)(threadIdx.x & 63);
+    for (int32_t e0 = blockIdx.x * 4; e0 < c[11]; e0 += gridDim.x * 4) {
+        const int32_t e = e0 + (int32_t)(threadIdx.x >> 6);
+        if (e >= c[11]) continue;
+        const int4 job = jobs[e];
+        if (job.w > kWaveJobMax) continue;  // uniform over the wave
         const int64_t first = job.z, end = (int64_t)job.z + job.w;
         DiskAcc acc;
         acc_init(acc);
-        // four faces per thread in flight: the loads do not wait on each other
-        for (int64_t s = first + lane; s < end; s += 4 * width) {
+        // four faces per lane in flight: the loads do not wait on each other
+        for (int64_t s = first + lane; s < end; s += 4 * 64) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (s + u * width < end) acc_face(acc, tris, s + u * width);
+                if (s + u * 64 < end) acc_face(acc, tris, s + u * 64);
         }
-        if (WAVE) {
-            for (int k = 0; k < 3; ++k) {
-                acc.lo[k] = wave_reduce(acc.lo[k], kMin);
-                acc.hi[k] = wave_reduce(acc.hi[k], kMax);
-                acc.n[k] = wave_reduce(acc.n[k], kSum);
-            }
-        } else {
-            block_reduce<BLOCK>(sh, acc.lo, 3, kMin);
-            block_reduce<BLOCK>(sh, acc.hi, 3, kMax);
-            block_reduce<BLOCK>(sh, acc.n, 3, kSum);
+        for (int k = 0; k < 3; ++k) {
+            acc.lo[k] = wave_reduce(acc.lo[k], kMin);
+            acc.hi[k] = wave_reduce(acc.hi[k], kMax);
+            acc.n[k] = wave_reduce(acc.n[k], kSum);
         }
         double cen[3];
         for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (acc.lo[k] + acc.hi[k]);
         const DiskFrame fr = frame_of(cen, acc.n);
         double ext[2] = {0.0, 0.0};
-        for (int64_t s = first + lane; s < end; s += 4 * width) {
+        for (int64_t s = first + lane; s < end; s += 4 * 64) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (s + u * width < end) {
-                    const double2 x = face_extent(fr, tris, s + u * width);
+                if (s + u * 64 < end) {
+                    const double2 x = face_extent(fr, tris, s + u * 64);
                     ext[0] = fmax(ext[0], x.x);
                     ext[1] = fmax(ext[1], x.y);
                 }
         }
-        if (WAVE) {
-            ext[0] = wave_reduce(ext[0], kMax);
-            ext[1] = wave_reduce(ext[1], kMax);
-        } else {
-            block_reduce<BLOCK>(sh, ext, 2, kMax);
-        }
+        ext[0] = wave_reduce(ext[0], kMax);
+        ext[1] = wave_reduce(ext[1], kMax);
         if (lane == 0) {
             store_disk(fr, ext[0], ext[1], cbnd + 8 * (int64_t)job.x + 2 * job.y);
             set_child_box(wn, job.x, job.y, acc.lo, acc.hi, abs_pad);
@@ -811,9 +924,118 @@ __global__ __launch_bounds__(BLOCK) void subtree_records(const int4* __restrict_
     }
 }
 
+// Children of more than kWaveJobMax faces: chunk y of big job j reduces its faces'
+// box and normal sum (pass 1) and, with the job's frame, their extents (pass 2) into
+// fixed slots; one wave per job folds the kChunks partials in a fixed order.
+__device__ __forceinline__ void chunk_range(const int4& job, int y, int64_t& b, int64_t& e) {
+    const int64_t per = ((int64_t)job.w + kChunks - 1) / kChunks;
+    b = (int64_t)job.z + y * per;
+    e = min((int64_t)job.z + job.w, b + per);
+}
+
+__global__ __launch_bounds__(256) void big_job_sums(const int4* __restrict__ jobs, const int32_t* __restrict__ big,
+                                                    const int32_t* c, const float4* __restrict__ tris,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[3 * 4];
+    for (int32_t j = (int32_t)blockIdx.x; j < c[9]; j += (int32_t)gridDim.x) {
+        const int4 job = jobs[big[j]];
+        int64_t b, e;
+        chunk_range(job, (int)blockIdx.y, b, e);
+        DiskAcc acc;
+        acc_init(acc);
+        for (int64_t s = b + threadIdx.x; s < e; s += 4 * 256) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (s + u * 256 < e) acc_face(acc, tris, s + u * 256);
+        }
+        block_reduce<256>(sh, acc.lo, 3, kMin);
+        block_reduce<256>(sh, acc.hi, 3, kMax);
+        block_reduce<256>(sh, acc.n, 3, kSum);
+        if (threadIdx.x == 0) {
+            double* o = part + 9 * ((int64_t)j * kChunks + blockIdx.y);
+            for (int k = 0; k < 3; ++k) {
+                o[k] = acc.lo[k];
+                o[3 + k] = acc.hi[k];
+                o[6 + k] = acc.n[k];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void big_job_frames(const int4* __restrict__ jobs, const int32_t* __restrict__ big,
+                                                      const int32_t* c, const double* __restrict__ part,
+                                                      DiskFrame* __restrict__ frames, WideNode* __restrict__ wn) {
+    const int lane = (int)(threadIdx.x & 63);
+    const double abs_pad = margin_of(c);
+    for (int32_t j = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6)); j < c[9]; j += (int32_t)(gridDim.x * 4)) {
+        const int4 job = jobs[big[j]];
+        double lo[3], hi[3], nn[3];
+        const double* q = part + 9 * ((int64_t)j * kChunks + lane);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = lane < kChunks ? q[k] : INFINITY;
+            hi[k] = lane < kChunks ? q[3 + k] : -INFINITY;
+            nn[k] = lane < kChunks ? q[6 + k] : 0.0;
+            lo[k] = wave_reduce(lo[k], kMin);
+            hi[k] = wave_reduce(hi[k], kMax);
+            nn[k] = wave_reduce(nn[k], kSum);
+        }
+        if (lane == 0) {
+            double cen[3];
+            for (int k = 0; k < 3; ++k) cen[k] = 0.5 * (lo[k] + hi[k]);
+            frames[j] = frame_of(cen, nn);
+            set_child_box(wn, job.x, job.y, lo, hi, abs_pad);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void big_job_extents(const int4* __restrict__ jobs, const int32_t* __restrict__ big,
+                                                       const int32_t* c, const float4* __restrict__ tris,
+                                                       const DiskFrame* __restrict__ frames,
+                                                       double* __restrict__ part) {
+    __shared__ double sh[3 * 4];
+    for (int32_t j = (int32_t)blockIdx.x; j < c[9]; j += (int32_t)gridDim.x) {
+        const int4 job = jobs[big[j]];
+        const DiskFrame fr = frames[j];
+        int64_t b, e;
+        chunk_range(job, (int)blockIdx.y, b, e);
+        double ext[2] = {0.0, 0.0};
+        for (int64_t s = b + threadIdx.x; s < e; s += 4 * 256) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (s + u * 256 < e) {
+                    const double2 x = face_extent(fr, tris, s + u * 256);
+                    ext[0] = fmax(ext[0], x.x);
+                    ext[1] = fmax(ext[1], x.y);
+                }
+        }
+        block_reduce<256>(sh, ext, 2, kMax);
+        if (threadIdx.x == 0) {
+            double* o = part + 2 * ((int64_t)j * kChunks + blockIdx.y);
+            o[0] = ext[0];
+            o[1] = ext[1];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void big_job_disks(const int4* __restrict__ jobs, const int32_t* __restrict__ big,
+                                                     const int32_t* c, const double* __restrict__ part,
+                                                     const DiskFrame* __restrict__ frames, float4* __restrict__ cbnd) {
+    const int lane = (int)(threadIdx.x & 63);
+    for (int32_t j = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6)); j < c[9]; j += (int32_t)(gridDim.x * 4)) {
+        const int4 job = jobs[big[j]];
+        const double* q = part + 2 * ((int64_t)j * kChunks + lane);
+        double r = lane < kChunks ? q[0] : 0.0, th = lane < kChunks ? q[1] : 0.0;
+        r = wave_reduce(r, kMax);
+        th = wave_reduce(th, kMax);
+        if (lane == 0) store_disk(frames[j], r, th, cbnd + 8 * (int64_t)job.x + 2 * job.y);
+    }
+}
+
 }  // namespace gb
 
 namespace {
+
+constexpr int kMaxDepth = 21;  // 3 * 21 + 1 = 64 stack entries
 
 size_t sort_tmp_bytes(int64_t n) {
     size_t a = 0, b = 0;
@@ -865,6 +1087,12 @@ size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
     w.flag_scan = take(n * sizeof(gb::U3));
     w.flag_tmp_bytes = flag_scan_bytes(n);
     w.flag_tmp = take(w.flag_tmp_bytes);
+    // children above kWaveJobMax faces: disjoint at each depth, so at most
+    // kMaxDepth * (n / (kWaveJobMax + 1) + 1) of them
+    const int64_t nbig = kMaxDepth * (n / (gb::kWaveJobMax + 1) + 1);
+    w.bigjobs = (int32_t*)take(nbig * sizeof(int32_t));
+    w.big_part = (double*)take(nbig * gb::kChunks * 9 * sizeof(double));
+    w.big_frames = take(nbig * sizeof(gb::DiskFrame));
     w.n = n;
     return off;
 }
@@ -884,7 +1112,6 @@ GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
 
 namespace {
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
-constexpr int kMaxDepth = 21;  // 3 * 21 + 1 = 64 stack entries
 }  // namespace
 
 // Median topology, steps 2-3: the three centroid-sorted face lists, then one stable
@@ -903,8 +1130,9 @@ hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w
                                                           0, 32, s);
         if (e != hipSuccess) return e;
     }
+    // global levels until every run fits one workgroup, then the rest in LDS
     int levels = 0;
-    for (int64_t run = n; run > leaf_size; run -= run / 2) ++levels;
+    for (int64_t run = n; run > gb::kFinishRun; run -= run / 2) ++levels;
     gb::U3* S = reinterpret_cast<gb::U3*>(w.flag_scan);
     for (int level = 0; level < levels; ++level) {
         const gb::Lists L{{cur[0], cur[1], cur[2]}};
@@ -919,7 +1147,10 @@ hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w
                            nxt[1], nxt[2], S, w.side);
         for (int a = 0; a < 3; ++a) std::swap(cur[a], nxt[a]);
     }
-    return hipMemcpyAsync(w.face_of, cur[0], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    const gb::Lists L{{cur[0], cur[1], cur[2]}};
+    hipLaunchKernelGGL(gb::median_finish, dim3(1u << levels), dim3(gb::kFinishThreads), 0, s, n, levels, leaf_size, L,
+                       w.cen, w.vals0, w.face_of);
+    return hipGetLastError();
 }
 
 hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, bool median,
@@ -961,10 +1192,11 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
         const int32_t* n_in = level == 0 ? (const int32_t*)nullptr : w.counters + 16 + level - 1;
         if (median)
             hipLaunchKernelGGL(gb::collapse_run_level, dim3(cblocks), dim3(256), 0, s, qin, n_in, qout,
-                               w.counters + 16 + level, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+                               w.counters + 16 + level, leaf_size, w.wide, w.leaves, w.jobs, w.bigjobs, w.counters);
         else
             hipLaunchKernelGGL(gb::collapse_level, dim3(cblocks), dim3(256), 0, s, qin, n_in, qout,
-                               w.counters + 16 + level, w.bn, leaf_size, w.wide, w.leaves, w.jobs, w.counters);
+                               w.counters + 16 + level, w.bn, leaf_size, w.wide, w.leaves, w.jobs, w.bigjobs,
+                               w.counters);
     }
     // every sorted face in exactly one leaf (leaf_parent is free again: reused as counts)
     hipLaunchKernelGGL(gb::fill_i32, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(256), 0, s, w.leaf_parent, n,
@@ -1000,10 +1232,16 @@ hipError_t gpu_build_arrays(int64_t n, const GpuBuildWorkspace& w, const TreeArr
                        const_cast<float4*>(out.tbnd), const_cast<float4*>(out.tpre));
     hipLaunchKernelGGL(gb::leaf_records, dim3(blocks_for(n)), dim3(256), 0, s, w.leaves, w.counters, out.tris,
                        const_cast<float4*>(out.lbnd), w.wide);
-    hipLaunchKernelGGL((gb::subtree_records<true, 256>), dim3(4096), dim3(256), 0, s, w.jobs, w.counters, out.tris,
+    hipLaunchKernelGGL(gb::subtree_records, dim3(4096), dim3(256), 0, s, w.jobs, w.counters, out.tris,
                        const_cast<float4*>(out.cbnd), w.wide);
-    hipLaunchKernelGGL((gb::subtree_records<false, 1024>), dim3(512), dim3(1024), 0, s, w.jobs, w.counters, out.tris,
-                       const_cast<float4*>(out.cbnd), w.wide);
+    const dim3 chunks(256, gb::kChunks);
+    hipLaunchKernelGGL(gb::big_job_sums, chunks, dim3(256), 0, s, w.jobs, w.bigjobs, w.counters, out.tris, w.big_part);
+    hipLaunchKernelGGL(gb::big_job_frames, dim3(64), dim3(256), 0, s, w.jobs, w.bigjobs, w.counters, w.big_part,
+                       reinterpret_cast<gb::DiskFrame*>(w.big_frames), w.wide);
+    hipLaunchKernelGGL(gb::big_job_extents, chunks, dim3(256), 0, s, w.jobs, w.bigjobs, w.counters, out.tris,
+                       reinterpret_cast<const gb::DiskFrame*>(w.big_frames), w.big_part);
+    hipLaunchKernelGGL(gb::big_job_disks, dim3(64), dim3(256), 0, s, w.jobs, w.bigjobs, w.counters, w.big_part,
+                       reinterpret_cast<const gb::DiskFrame*>(w.big_frames), const_cast<float4*>(out.cbnd));
     return hipGetLastError();
 }
 

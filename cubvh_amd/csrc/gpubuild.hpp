@@ -19,7 +19,7 @@ struct BinNode {
 struct GpuBuildWorkspace {
     int32_t* counters = nullptr;  // [0..5] centroid bounds (ordered ints), [6] max |coordinate| bits,
                                   // [7] input errors (1 bad index, 2 non-finite), [8] wide nodes,
-                                  // [10] leaves, [11] subtree jobs, [12] depth, [13] slot of
+                                  // [9] jobs above 4,096 faces, [10] leaves, [11] subtree jobs, [12] depth, [13] slot of
                                   // face 0, [14] structure error, [16 + k] entries made by
                                   // collapse level k; then the gather partials
     float* tri9 = nullptr;        // [9n] gathered vertices, original face order
@@ -43,6 +43,10 @@ struct GpuBuildWorkspace {
     void* flag_scan = nullptr;    // [n] scanned lower-half flags of the three lists
     void* flag_tmp = nullptr;
     size_t flag_tmp_bytes = 0;
+    // subtrees above 4,096 faces, reduced in chunks
+    int32_t* bigjobs = nullptr;   // job indices, counted in counters[9]
+    double* big_part = nullptr;   // chunk partials
+    void* big_frames = nullptr;   // disk frame per big job
     int64_t n = 0;
 };
 
