@@ -1186,7 +1186,14 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (e == hipSuccess) e = hipMemcpyAsync(w.counters + 8, &one, sizeof one, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     const unsigned cblocks = std::min<unsigned>(blocks_for(n), 1024);
-    for (int level = 0; level < kMaxDepth; ++level) {
+    // the median tree's depth follows from n: two halvings per wide level
+    int collapse_levels = kMaxDepth;
+    if (median) {
+        int halvings = 0;
+        for (int64_t run = n; run > leaf_size; run -= run / 2) ++halvings;
+        collapse_levels = std::min(kMaxDepth, (halvings + 1) / 2 + 1);
+    }
+    for (int level = 0; level < collapse_levels; ++level) {
         const int4* qin = (level & 1) ? w.queue1 : w.queue0;
         int4* qout = (level & 1) ? w.queue0 : w.queue1;
         const int32_t* n_in = level == 0 ? (const int32_t*)nullptr : w.counters + 16 + level - 1;
@@ -1208,7 +1215,7 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     res.input_errors = c[7];
-    res.too_deep = c[16 + kMaxDepth - 1] != 0;
+    res.too_deep = c[16 + collapse_levels - 1] != 0;
     res.structure_ok = c[14] == 0;
     res.n_nodes = c[8];
     res.n_leaves = c[10];
