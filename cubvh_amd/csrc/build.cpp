@@ -8,6 +8,10 @@
 //      <= leaf_size triangles, building independent subtrees on worker threads;
 //   2. collapses it into 4-wide nodes by repeatedly opening the child with the
 //      largest surface area, so every wide node has up to 4 children;
+//      the areas are those of the boxes grown by kDilate of the scene diagonal on every
+//      side: a distance query visits a box when its search ball reaches it, so the
+//      chance of a visit follows the area of the box widened by a typical ball radius
+//      (measured on the 256^3 grid: +1.9% unsigned, +1.5% signed distance, rays equal);
 //   3. writes nodes in depth-first pre-order and triangles in leaf order.
 // Trees deeper than the traversal stack allows (3 * depth + 1 > 64 entries) are
 // refused by the caller; median splits always stay far below that.
@@ -39,9 +43,13 @@ struct Aabb {
             hi[k] = std::max(hi[k], o.hi[k]);
         }
     }
-    float area() const {
+    // surface area of the box grown by `grow` (total, over both sides) on every axis
+    float area(float grow) const {
         float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
         if (d0 < 0) return 0;
+        d0 += grow;
+        d1 += grow;
+        d2 += grow;
         return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
     }
 };
@@ -52,6 +60,9 @@ struct BNode {
     int64_t first = 0;
     int32_t count = 0;
 };
+
+// Growth of the boxes in the area terms, as a fraction of the scene diagonal.
+constexpr float kDilate = 0.01f;
 
 struct Builder {
     const float* tri9;
@@ -91,6 +102,7 @@ struct Builder {
     }
 
     int split_mode = 0;  // 0 object median, 1 binned SAH
+    float grow = 0.0f;   // box growth in the area terms
 
     int64_t split(int64_t b, int64_t e) { return split_mode == 1 ? split_sah(b, e) : split_median(b, e); }
 
@@ -136,7 +148,7 @@ struct Builder {
                 left.grow(bb[k]);
                 lc += cnt[k];
                 if (lc == 0 || rc[k + 1] == 0) continue;
-                float cost = left.area() * (float)lc + right[k + 1].area() * (float)rc[k + 1];
+                float cost = left.area(grow) * (float)lc + right[k + 1].area(grow) * (float)rc[k + 1];
                 if (cost < best_cost) {
                     best_cost = cost;
                     best_axis = axis;
@@ -229,6 +241,12 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
         B.idx[i] = i;
     }
 
+    {
+        const Aabb all = B.range_box(0, n_faces);
+        float diag2 = 0.0f;
+        for (int k = 0; k < 3; ++k) diag2 += (all.hi[k] - all.lo[k]) * (all.hi[k] - all.lo[k]);
+        B.grow = kDilate * std::sqrt(diag2);
+    }
     // Top levels sequentially until there are enough independent ranges, then one
     // worker per range.
     unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -328,7 +346,7 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
             float ba = -1.0f;
             for (size_t k = 0; k < kids.size(); ++k) {
                 const BNode& x = bn[kids[k]];
-                if (x.left >= 0 && x.box.area() > ba) { ba = x.box.area(); best = (int)k; }
+                if (x.left >= 0 && x.box.area(B.grow) > ba) { ba = x.box.area(B.grow); best = (int)k; }
             }
             if (best < 0) break;
             int32_t o = kids[best];
