@@ -397,7 +397,8 @@ def run_extras(dev, steps, warmup, out):
     import ctypes
 
     stream = torch.cuda.current_stream(dev).cuda_stream
-    for name, flags, build in (("gpu_build_level8", 0, "gpu"), ("gpu_build_level8_lbvh", 0x100, "gpu_lbvh")):
+    for name, flags, build in (("gpu_build_level8", 0, "gpu"), ("gpu_build_level8_sah", 0x200, "gpu_sah"),
+                               ("gpu_build_level8_lbvh", 0x100, "gpu_lbvh")):
         times = []
         for _ in range(3):
             h = ctypes.c_void_p()

@@ -61,8 +61,10 @@ class cuBVH:
     build="host" (default): binned-SAH build on the host, no GPU touched; the tree is
     copied to a GPU on first use.  build="gpu": built on `device` (default the current
     CUDA device) by object-median splits on each run's longest centroid axis, level by
-    level; build="gpu_lbvh": the Morton radix tree (linear BVH), the fastest build and
-    the slowest to query.  Query results are the same whatever the build.
+    level; build="gpu_sah": every run split where the surface-area cost over the sorted
+    face lists is least, the slowest GPU build and the fastest to query; build="gpu_lbvh":
+    the Morton radix tree (linear BVH), the fastest build and the slowest to query.
+    Query results are the same whatever the build.
     """
 
     def __init__(self, vertices, triangles, leaf_size: int = 0, build: str = "host", device=None):
@@ -74,15 +76,15 @@ class cuBVH:
         if build == "host":
             _lib.check(lib.cubvh_bvh_create(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0], int(leaf_size),
                                             ctypes.byref(h)))
-        elif build in ("gpu", "gpu_lbvh"):
+        elif build in ("gpu", "gpu_sah", "gpu_lbvh"):
             dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
             vt = torch.from_numpy(v).to(dev)
             ft = torch.from_numpy(f).to(dev)
-            flags = int(leaf_size) | (0x100 if build == "gpu_lbvh" else 0)
+            flags = int(leaf_size) | {"gpu": 0, "gpu_lbvh": 0x100, "gpu_sah": 0x200}[build]
             _lib.check(lib.cubvh_bvh_create_gpu(dev.index, vt.data_ptr(), v.shape[0], ft.data_ptr(), f.shape[0],
                                                 flags, ctypes.byref(h), _stream_of(vt)))
         else:
-            raise ValueError(f"build must be 'host', 'gpu' or 'gpu_lbvh', not {build!r}")
+            raise ValueError(f"build must be 'host', 'gpu', 'gpu_sah' or 'gpu_lbvh', not {build!r}")
         self._h = h
         self._lib = lib
         self._faces = f

@@ -435,8 +435,10 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "mesh must have vertices and faces");
     if (n_faces > MAX_LEAF_FIRST || n_vertices > (int64_t)0x7fffffff)
         return fail(CUBVH_ERR_ARG, "too many faces or vertices for the GPU build");
-    if (leaf_size < 0 || (leaf_size & ~0x1ff)) return fail(CUBVH_ERR_ARG, "unknown bits in leaf_size");
-    const bool median = (leaf_size & 0x100) == 0;  // bit 8: Morton radix tree instead
+    if (leaf_size < 0 || (leaf_size & ~0x3ff)) return fail(CUBVH_ERR_ARG, "unknown bits in leaf_size");
+    if ((leaf_size & 0x300) == 0x300) return fail(CUBVH_ERR_ARG, "leaf_size bits 8 and 9 exclude each other");
+    // bit 8: Morton radix tree, bit 9: surface-area heuristic, neither: object median
+    const int topology = (leaf_size & 0x100) ? kTopologyLinear : (leaf_size & 0x200) ? kTopologySah : kTopologyMedian;
     leaf_size &= 0xff;
     if (leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..8");
     if (leaf_size == 0) leaf_size = kDefaultLeaf;
@@ -457,7 +459,7 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     int rc = ensure_pool(device);
     if (rc != CUBVH_OK) return rc;
     void* ws = nullptr;
-    hipError_t e = hipMallocAsync(&ws, gpu_build_workspace_bytes(n_faces), hs);
+    hipError_t e = hipMallocAsync(&ws, gpu_build_workspace_bytes(n_faces, topology), hs);
     if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(build workspace)");
     struct WsGuard {
         void* p;
@@ -467,9 +469,15 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
             (void)hipStreamSynchronize(s);
         }
     } guard{ws, hs};
-    const GpuBuildWorkspace w = carve_gpu_build_workspace(ws, n_faces);
+    const GpuBuildWorkspace w = carve_gpu_build_workspace(ws, n_faces, topology);
     GpuBuildResult res;
-    e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, median, w, res, hs);
+    e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, topology, w, res, hs);
+    if (e == hipSuccess && topology == kTopologySah && res.too_deep && res.input_errors == 0) {
+        // an SAH tree deeper than the stack accepts: the median tree (the host build's
+        // fallback too); the SAH workspace holds everything the median build uses
+        res = GpuBuildResult();
+        e = gpu_build_tree(vertices, n_vertices, faces, n_faces, leaf_size, kTopologyMedian, w, res, hs);
+    }
     if (e != hipSuccess) return fail_hip(e, "GPU tree build");
     if (res.input_errors & 1) return fail(CUBVH_ERR_ARG, "a face references a vertex outside [0, n_vertices)");
     if (res.input_errors & 2) return fail(CUBVH_ERR_ARG, "vertex coordinates must be finite");

@@ -5,16 +5,20 @@
 //   1. faces    : gather each face's vertices, check indices and coordinates, bounds
 //                 of the centroids and the largest |coordinate| (order-preserving
 //                 int atomics);
-//   2. keys     : 63-bit Morton code of each centroid (21 bits per axis) and a radix
-//                 sort of (code, face) pairs;
-//   3. tree     : the binary radix tree over the sorted codes (Karras, "Maximizing
-//                 parallelism in the construction of BVHs, octrees and k-d trees",
-//                 HPG 2012), equal codes ordered by position; one thread per
-//                 internal node;
+//   2-3. topology, one of
+//        median : three centroid-sorted face lists, every run halved level by level
+//                 at the median of its axis of largest centroid extent (below);
+//        SAH    : the same lists, every run split at the slot of least surface-area
+//                 cost, from segmented scans of the face boxes (below);
+//        linear : 63-bit Morton code of each centroid (21 bits per axis), a radix
+//                 sort of (code, face) pairs, and the binary radix tree over the
+//                 sorted codes (Karras, "Maximizing parallelism in the construction
+//                 of BVHs, octrees and k-d trees", HPG 2012), equal codes ordered by
+//                 position; one thread per internal node;
 //   4. collapse : top-down, one launch per level: a wide node takes its binary
-//                 node's two children and opens the child with the largest Morton
-//                 cell (shortest common prefix of its codes) until it has 4;
-//                 subtrees of <= leaf_size faces become leaves;
+//                 node's two children and opens the child that comes first (the
+//                 largest Morton cell, the largest box area, or the longer median
+//                 run) until it has 4; subtrees of <= leaf_size faces become leaves;
 //   5. arrays   : leaf-ordered triangles; then one pass over each child's run of
 //                 faces gives its box (written into its wide node) and its bounding
 //                 disk; face disks and the per-face constants of the distance
@@ -439,6 +443,200 @@ __global__ __launch_bounds__(kFinishThreads) void median_finish(int64_t n, int g
         cur ^= 1;
     }
     for (int i = (int)threadIdx.x; i < k; i += kFinishThreads) face_of[b + i] = face[lst[cur][0][i]];
+}
+
+// ---- SAH topology
+// Every run of more than leaf_size slots [f, l] is split at the position of least
+// surface-area cost over all three centroid-sorted lists: for the split before slot k
+// of list a, cost = A(faces at [f, k) of list a) * (k - f) + A(faces at [k, l]) * (l + 1 - k),
+// A the surface area of the faces' bounding box grown by `grow` (1% of the scene
+// diagonal, as the host build) on every axis.  The boxes of every prefix and suffix of
+// every run come from segmented scans of the face boxes (by run), one level of the
+// tree per round; the chosen split partitions all three lists stably, as the median
+// topology does.  The binary nodes (every node, leaves included) go to `bn`, which the
+// radix-tree collapse then turns into wide nodes; a node's prefix field holds its own
+// box area made into an ordering key (smaller = larger box), so that the collapse opens
+// the child of largest surface area first, as the host collapse does.  Below level
+// kSahLevels the splits fall back to the median on the axis of largest centroid extent,
+// so every run ends within a bounded number of levels.
+constexpr float kSahDilate = 0.01f;
+constexpr int kSahLevels = 40;
+constexpr int kSahMaxLevels = 96;
+
+struct Box6 {
+    float lx, ly, lz, hx, hy, hz;
+};
+struct BoxUnion {
+    __host__ __device__ Box6 operator()(const Box6& a, const Box6& b) const {
+        return Box6{fminf(a.lx, b.lx), fminf(a.ly, b.ly), fminf(a.lz, b.lz),
+                    fmaxf(a.hx, b.hx), fmaxf(a.hy, b.hy), fmaxf(a.hz, b.hz)};
+    }
+};
+// Run key and face box of slot i of list `list`, read forward or from the end.
+struct RunAt {
+    const int32_t* rid;
+    int64_t n;
+    bool rev;
+    __host__ __device__ int32_t operator()(int64_t i) const { return rid[rev ? n - 1 - i : i]; }
+};
+struct BoxAt {
+    const uint32_t* list;
+    const Box6* fbox;
+    int64_t n;
+    bool rev;
+    __host__ __device__ Box6 operator()(int64_t i) const { return fbox[list[rev ? n - 1 - i : i]]; }
+};
+
+__device__ __forceinline__ float grown_area(const Box6& b, float g) {
+    const float d0 = b.hx - b.lx + g, d1 = b.hy - b.ly + g, d2 = b.hz - b.lz + g;
+    return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
+}
+
+// Face boxes; every slot starts in the root run (node 0).
+__global__ __launch_bounds__(256) void sah_boxes(int64_t n, const float* __restrict__ tri9, Box6* __restrict__ fbox,
+                                                 int32_t* __restrict__ rid) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const float* t = tri9 + 9 * f;
+    fbox[f] = Box6{fminf(fminf(t[0], t[3]), t[6]), fminf(fminf(t[1], t[4]), t[7]), fminf(fminf(t[2], t[5]), t[8]),
+                   fmaxf(fmaxf(t[0], t[3]), t[6]), fmaxf(fmaxf(t[1], t[4]), t[7]), fmaxf(fmaxf(t[2], t[5]), t[8])};
+    rid[f] = 0;
+}
+
+// The root node [0, n - 1], active at level 0.  cnt[] is zero on entry.
+__global__ void sah_root(int64_t n, BinNode* bn, int32_t* act, int32_t* cnt, unsigned long long* best, int64_t cap,
+                         int32_t* c) {
+    if (threadIdx.x != 0) return;
+    bn[0] = BinNode{-1, -1, 0, (int32_t)(n - 1), -1, 0};
+    act[0] = 0;
+    cnt[0] = 1;
+    c[40] = 1;
+    for (int a = 0; a < 3; ++a) best[a * cap] = ~0ull;
+}
+
+// grow = kSahDilate * diagonal of the box of all faces (the root run's full prefix).
+__global__ void sah_grow(const Box6* __restrict__ last, float* grow) {
+    if (threadIdx.x != 0) return;
+    const Box6 b = *last;
+    const float dx = b.hx - b.lx, dy = b.hy - b.ly, dz = b.hz - b.lz;
+    *grow = kSahDilate * sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// Cost of the split before slot p + 1 of list a, for every slot p of an active run but
+// its last; the least (cost, position) of each run goes to best[run] (one atomic per
+// run and wave: the runs of a wave are contiguous slots).
+__global__ __launch_bounds__(256) void sah_cost(int64_t n, const int32_t* __restrict__ rid,
+                                                const BinNode* __restrict__ bn, const Box6* __restrict__ pre,
+                                                const Box6* __restrict__ sufrev, const float* __restrict__ grow,
+                                                unsigned long long* __restrict__ best) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t r = p < n ? rid[p] : -1;
+    unsigned long long key = ~0ull;
+    if (r >= 0) {
+        const int32_t f = bn[r].first, l = bn[r].last;
+        if (p < l) {
+            const float g = *grow;
+            const float cost = grown_area(pre[p], g) * (float)(p - f + 1) +
+                               grown_area(sufrev[n - 2 - p], g) * (float)(l - p);
+            key = ((unsigned long long)__float_as_uint(cost) << 32) | (uint32_t)(p + 1);
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t klo = __shfl_down((uint32_t)key, off), khi = __shfl_down((uint32_t)(key >> 32), off);
+        const int32_t ro = __shfl_down(r, off);
+        const unsigned long long o = ((unsigned long long)khi << 32) | klo;
+        if (lane + off < 64 && ro == r && o < key) key = o;
+    }
+    const int32_t rprev = __shfl_up(r, 1);
+    if (r >= 0 && (lane == 0 || rprev != r) && key != ~0ull) atomicMin(best + r, key);
+}
+
+// One active run per thread: its split (axis << 28 | slot), its two child nodes, the
+// children still longer than leaf_size to the next level's list.
+__global__ __launch_bounds__(256) void sah_choose(int level, int32_t* __restrict__ cnt, const int32_t* __restrict__ act_in,
+                                                  int32_t* __restrict__ act_out, BinNode* __restrict__ bn,
+                                                  unsigned long long* __restrict__ best, int64_t cap,
+                                                  uint32_t* __restrict__ split, const Box6* __restrict__ pre,
+                                                  const float* __restrict__ grow, Lists L,
+                                                  const float* __restrict__ cen, int64_t n, int32_t leaf, int32_t* c) {
+    const int32_t m = cnt[level];
+    for (int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); e < m; e += (int32_t)(gridDim.x * blockDim.x)) {
+        const int32_t r = act_in[e];
+        const int32_t f = bn[r].first, l = bn[r].last;
+        unsigned long long kb = ~0ull;
+        int axis = 0;
+        for (int a = 0; a < 3; ++a) {
+            const unsigned long long x = best[a * cap + r];
+            if (x < kb) {
+                kb = x;
+                axis = a;
+            }
+        }
+        int32_t k = (int32_t)(uint32_t)kb;
+        if (level >= kSahLevels || kb == ~0ull || k <= f || k > l) {
+            axis = run_axis(L.l, cen, n, f, (int64_t)l + 1);
+            k = f + (l + 1 - f) / 2;
+        }
+        split[r] = ((uint32_t)axis << 28) | (uint32_t)k;
+        bn[r].prefix = 0x3fffffff - (int32_t)(__float_as_uint(grown_area(pre[l], *grow)) >> 1);
+        const int32_t id = atomicAdd(&c[40], 2);
+        bn[r].left = id;
+        bn[r].right = id + 1;
+        bn[id] = BinNode{-1, -1, f, k - 1, r, 0};
+        bn[id + 1] = BinNode{-1, -1, k, l, r, 0};
+        const int32_t sz[2] = {k - f, l + 1 - k};
+        for (int h = 0; h < 2; ++h) {
+            if (sz[h] <= leaf) continue;
+            act_out[atomicAdd(&cnt[level + 1], 1)] = id + h;
+            for (int a = 0; a < 3; ++a) best[a * cap + id + h] = ~0ull;
+        }
+    }
+}
+
+// side[f] = 0 for the faces before the split slot of the run's chosen list, 1 after.
+__global__ __launch_bounds__(256) void sah_mark(int64_t n, const int32_t* __restrict__ rid,
+                                                const uint32_t* __restrict__ split, Lists L,
+                                                uint8_t* __restrict__ side) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t r = rid[p];
+    if (r < 0) return;
+    const uint32_t s = split[r];
+    side[L.l[s >> 28][p]] = p < (int64_t)(s & 0x0fffffffu) ? 0 : 1;
+}
+
+// Stable partition of every active run of each list by side (S: exclusive scan of the
+// flags), and every slot's run for the next level (-1 once its run is a leaf).
+__global__ __launch_bounds__(256) void sah_scatter(int64_t n, int32_t* __restrict__ rid,
+                                                   const BinNode* __restrict__ bn,
+                                                   const uint32_t* __restrict__ split, Lists L, uint32_t* N0,
+                                                   uint32_t* N1, uint32_t* N2, const U3* __restrict__ S,
+                                                   const uint8_t* __restrict__ side, int32_t leaf, int32_t* c) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t* out[3] = {N0, N1, N2};
+    const int32_t r = rid[p];
+    if (r < 0) {
+        for (int a = 0; a < 3; ++a) out[a][p] = L.l[a][p];
+        return;
+    }
+    const BinNode nd = bn[r];
+    const int64_t b = nd.first, k = (int64_t)(split[r] & 0x0fffffffu);
+    const U3 sp = S[p], sb = S[b];
+    const uint32_t below[3] = {sp.x - sb.x, sp.y - sb.y, sp.z - sb.z};
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t f = L.l[a][p];
+        const int64_t d = side[f] == 0 ? b + below[a] : k + (p - b - below[a]);
+        if (d < b || d > nd.last) {  // the lists disagree on the run's faces: never for a sound build
+            c[14] = 1;
+            continue;
+        }
+        out[a][d] = f;
+    }
+    const bool lower = p < k;
+    const int64_t size = lower ? k - b : (int64_t)nd.last + 1 - k;
+    rid[p] = size > leaf ? (lower ? nd.left : nd.right) : -1;
 }
 
 // Outward rounding by a relative and an absolute margin (the host build's rule).
@@ -1047,6 +1245,16 @@ size_t sort_tmp_bytes(int64_t n) {
 }
 
 using FlagIter = hipcub::TransformInputIterator<gb::U3, gb::LowerFlags, hipcub::CountingInputIterator<int64_t>>;
+using RunIter = hipcub::TransformInputIterator<int32_t, gb::RunAt, hipcub::CountingInputIterator<int64_t>>;
+using BoxIter = hipcub::TransformInputIterator<gb::Box6, gb::BoxAt, hipcub::CountingInputIterator<int64_t>>;
+
+size_t box_scan_bytes(int64_t n) {
+    size_t b = 0;
+    RunIter keys(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{});
+    BoxIter vals(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{});
+    hipcub::DeviceScan::InclusiveScanByKey(nullptr, b, keys, vals, (gb::Box6*)nullptr, gb::BoxUnion(), (int)n);
+    return b;
+}
 
 size_t flag_scan_bytes(int64_t n) {
     size_t b = 0;
@@ -1056,7 +1264,7 @@ size_t flag_scan_bytes(int64_t n) {
 }
 
 // The workspace layout, measured (ws == nullptr) or carved.
-size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
+size_t lay_out_workspace(void* ws, int64_t n, int topology, GpuBuildWorkspace& w) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const int64_t ni = n > 1 ? n - 1 : 1;
     char* base = (char*)ws;
@@ -1074,7 +1282,8 @@ size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
     w.face_of = (uint32_t*)take(n * sizeof(uint32_t));
     w.sort_tmp_bytes = sort_tmp_bytes(n);
     w.sort_tmp = take(w.sort_tmp_bytes);
-    w.bn = (BinNode*)take(ni * sizeof(BinNode));
+    const bool sah = topology == kTopologySah;
+    w.bn = (BinNode*)take((sah ? 2 * n : ni) * sizeof(BinNode));
     w.leaf_parent = (int32_t*)take(n * sizeof(int32_t));
     w.wide = (WideNode*)take(ni * sizeof(WideNode));
     w.queue0 = (int4*)take(ni * sizeof(int4));
@@ -1087,6 +1296,20 @@ size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
     w.flag_scan = take(n * sizeof(gb::U3));
     w.flag_tmp_bytes = flag_scan_bytes(n);
     w.flag_tmp = take(w.flag_tmp_bytes);
+    if (sah) {
+        w.fbox = (float*)take(6 * n * sizeof(float));
+        w.rid = (int32_t*)take(n * sizeof(int32_t));
+        w.best = (unsigned long long*)take(3 * 2 * n * sizeof(unsigned long long));
+        w.split = (uint32_t*)take(2 * n * sizeof(uint32_t));
+        w.act0 = (int32_t*)take(n * sizeof(int32_t));
+        w.act1 = (int32_t*)take(n * sizeof(int32_t));
+        w.level_count = (int32_t*)take((gb::kSahMaxLevels + 1) * sizeof(int32_t));
+        w.box_pre = (float*)take(6 * n * sizeof(float));
+        w.box_suf = (float*)take(6 * n * sizeof(float));
+        w.grow = (float*)take(sizeof(float));
+        w.box_tmp_bytes = box_scan_bytes(n);
+        w.box_tmp = take(w.box_tmp_bytes);
+    }
     // children above kWaveJobMax faces: disjoint at each depth, so at most
     // kMaxDepth * (n / (kWaveJobMax + 1) + 1) of them
     const int64_t nbig = kMaxDepth * (n / (gb::kWaveJobMax + 1) + 1);
@@ -1099,14 +1322,14 @@ size_t lay_out_workspace(void* ws, int64_t n, GpuBuildWorkspace& w) {
 
 }  // namespace
 
-size_t gpu_build_workspace_bytes(int64_t n) {
+size_t gpu_build_workspace_bytes(int64_t n, int topology) {
     GpuBuildWorkspace w;
-    return lay_out_workspace(nullptr, n, w);
+    return lay_out_workspace(nullptr, n, topology, w);
 }
 
-GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n) {
+GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n, int topology) {
     GpuBuildWorkspace w;
-    lay_out_workspace(ws, n, w);
+    lay_out_workspace(ws, n, topology, w);
     return w;
 }
 
@@ -1114,13 +1337,10 @@ namespace {
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
 }  // namespace
 
-// Median topology, steps 2-3: the three centroid-sorted face lists, then one stable
-// partition of every run per level until all runs are leaves; the first list is the
-// leaf order.
-hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, hipStream_t s) {
+// The centroids and the three centroid-sorted face lists (by (centroid, face index))
+// in cur[0..2].
+hipError_t sorted_lists(int64_t n, const GpuBuildWorkspace& w, uint32_t* const* cur, hipStream_t s) {
     hipLaunchKernelGGL(gb::centroids, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.cen);
-    uint32_t* cur[3] = {w.lists, w.lists + n, w.lists + 2 * n};
-    uint32_t* nxt[3] = {w.lists + 3 * n, w.lists + 4 * n, w.lists + 5 * n};
     uint32_t* keys = reinterpret_cast<uint32_t*>(w.keys0);
     uint32_t* keys_sorted = reinterpret_cast<uint32_t*>(w.keys1);
     for (int a = 0; a < 3; ++a) {
@@ -1130,6 +1350,17 @@ hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w
                                                           0, 32, s);
         if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+// Median topology, steps 2-3: the three centroid-sorted face lists, then one stable
+// partition of every run per level until all runs are leaves; the first list is the
+// leaf order.
+hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, hipStream_t s) {
+    uint32_t* cur[3] = {w.lists, w.lists + n, w.lists + 2 * n};
+    uint32_t* nxt[3] = {w.lists + 3 * n, w.lists + 4 * n, w.lists + 5 * n};
+    hipError_t e0 = sorted_lists(n, w, cur, s);
+    if (e0 != hipSuccess) return e0;
     // global levels until every run fits one workgroup, then the rest in LDS
     int levels = 0;
     for (int64_t run = n; run > gb::kFinishRun; run -= run / 2) ++levels;
@@ -1153,8 +1384,87 @@ hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w
     return hipGetLastError();
 }
 
-hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, bool median,
+// SAH topology, steps 2-3: the sorted lists, then one level of binary nodes per round
+// (segmented box scans of each list, the least-cost split of every active run, one
+// stable partition of the three lists) until no run is longer than leaf_size; the
+// binary nodes in w.bn, the first list is the leaf order.  The host reads the count of
+// active runs after the levels a median tree would need and then every 4 levels;
+// finished = false if runs are still active after kSahMaxLevels levels (cannot happen:
+// splits fall back to the median below kSahLevels).
+hipError_t sah_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, bool& finished, hipStream_t s) {
+    finished = false;
+    uint32_t* cur[3] = {w.lists, w.lists + n, w.lists + 2 * n};
+    uint32_t* nxt[3] = {w.lists + 3 * n, w.lists + 4 * n, w.lists + 5 * n};
+    hipError_t e = sorted_lists(n, w, cur, s);
+    if (e != hipSuccess) return e;
+    const int64_t cap = 2 * n;
+    gb::Box6* fbox = reinterpret_cast<gb::Box6*>(w.fbox);
+    gb::Box6* pre = reinterpret_cast<gb::Box6*>(w.box_pre);
+    gb::Box6* suf = reinterpret_cast<gb::Box6*>(w.box_suf);
+    hipLaunchKernelGGL(gb::sah_boxes, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, fbox, w.rid);
+    hipLaunchKernelGGL(gb::fill_i32, dim3(1), dim3(256), 0, s, w.level_count, (int64_t)gb::kSahMaxLevels + 1, 0);
+    hipLaunchKernelGGL(gb::sah_root, dim3(1), dim3(64), 0, s, n, w.bn, w.act0, w.level_count, w.best, cap,
+                       w.counters);
+    int32_t* act_in = w.act0;
+    int32_t* act_out = w.act1;
+    gb::U3* S = reinterpret_cast<gb::U3*>(w.flag_scan);
+    int halvings = 0;
+    for (int64_t run = n; run > leaf_size; run -= run / 2) ++halvings;
+    int level = 0, batch = halvings + 2;
+    const int64_t max_active = n / (leaf_size + 1) + 1;
+    for (;;) {
+        for (int i = 0; i < batch && level < gb::kSahMaxLevels; ++i, ++level) {
+            const gb::Lists L{{cur[0], cur[1], cur[2]}};
+            for (int a = 0; a < 3; ++a) {
+                size_t tmp = w.box_tmp_bytes;
+                RunIter kf(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{w.rid, n, false});
+                BoxIter vf(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{cur[a], fbox, n, false});
+                e = hipcub::DeviceScan::InclusiveScanByKey(w.box_tmp, tmp, kf, vf, pre, gb::BoxUnion(), (int)n,
+                                                           hipcub::Equality(), s);
+                if (e != hipSuccess) return e;
+                if (level == 0 && a == 0) hipLaunchKernelGGL(gb::sah_grow, dim3(1), dim3(64), 0, s, pre + (n - 1), w.grow);
+                tmp = w.box_tmp_bytes;
+                RunIter kr(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{w.rid, n, true});
+                BoxIter vr(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{cur[a], fbox, n, true});
+                e = hipcub::DeviceScan::InclusiveScanByKey(w.box_tmp, tmp, kr, vr, suf, gb::BoxUnion(), (int)n,
+                                                           hipcub::Equality(), s);
+                if (e != hipSuccess) return e;
+                hipLaunchKernelGGL(gb::sah_cost, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.bn, pre, suf,
+                                   w.grow, w.best + a * cap);
+            }
+            const int64_t active = std::min<int64_t>(level < 30 ? (int64_t)1 << level : max_active, max_active);
+            hipLaunchKernelGGL(gb::sah_choose, dim3(std::min<unsigned>(blocks_for(active), 1024)), dim3(256), 0, s,
+                               level, w.level_count, act_in, act_out, w.bn, w.best, cap, w.split, pre, w.grow, L,
+                               w.cen, n, leaf_size, w.counters);
+            hipLaunchKernelGGL(gb::sah_mark, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.split, L, w.side);
+            FlagIter flags(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{L, w.side});
+            size_t tmp = w.flag_tmp_bytes;
+            e = hipcub::DeviceScan::ExclusiveScan(w.flag_tmp, tmp, flags, S, gb::U3Sum(), gb::U3{0, 0, 0}, (int)n, s);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(gb::sah_scatter, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.bn, w.split, L,
+                               nxt[0], nxt[1], nxt[2], S, w.side, leaf_size, w.counters);
+            for (int a = 0; a < 3; ++a) std::swap(cur[a], nxt[a]);
+            std::swap(act_in, act_out);
+        }
+        int32_t left = 0;
+        e = hipMemcpyAsync(&left, w.level_count + level, sizeof left, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        if (left == 0) break;
+        if (level >= gb::kSahMaxLevels) return hipGetLastError();
+        batch = 4;
+    }
+    finished = true;
+    e = hipMemcpyAsync(w.face_of, cur[0], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, int topology,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s) {
+    const bool median = topology == kTopologyMedian;
+    const bool sah = topology == kTopologySah;
+    bool sah_finished = true;
     hipError_t e = hipSuccess;
     hipLaunchKernelGGL(gb::init_counters, dim3(1), dim3(64), 0, s, w.counters);
     const unsigned gblocks = std::min<unsigned>(blocks_for(n), gb::kGatherBlocks);
@@ -1163,6 +1473,18 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (median) {
         e = median_order(n, leaf_size, w, s);
         if (e != hipSuccess) return e;
+    } else if (sah) {
+        e = sah_order(n, leaf_size, w, sah_finished, s);
+        if (e != hipSuccess) return e;
+        if (!sah_finished) {  // no collapse of an unfinished tree
+            int32_t c[16];
+            e = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            res.input_errors = c[7];
+            res.too_deep = true;
+            return hipGetLastError();
+        }
     } else {
         hipLaunchKernelGGL(gb::morton_keys, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, w.counters, w.keys0,
                            w.vals0);
@@ -1178,7 +1500,7 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     // k reads its entry count from counters[16 + k - 1] (the root alone at level 0) and
     // counts its outputs in counters[16 + k]; a tree still growing after the last
     // level is reported as too deep.  The root entry is (binary root 0, wide node 0,
-    // depth 1) for the radix tree, (slots 0 .. n, depth 1, wide node 0) for the median
+    // depth 1) for the radix and SAH trees, (slots 0 .. n, depth 1, wide node 0) for the median
     // topology.
     const int4 root = median ? make_int4(0, (int32_t)n, 1, 0) : make_int4(0, 0, 1, 0);
     const int32_t one = 1;
@@ -1215,7 +1537,7 @@ hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t 
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     res.input_errors = c[7];
-    res.too_deep = c[16 + collapse_levels - 1] != 0;
+    res.too_deep = c[16 + collapse_levels - 1] != 0 || !sah_finished;
     res.structure_ok = c[14] == 0;
     res.n_nodes = c[8];
     res.n_leaves = c[10];

@@ -9,11 +9,20 @@
 
 namespace cubvh_amd {
 
-// Binary radix-tree node: children >= 0 are internal nodes, < 0 the leaf ~j (sorted
-// face j); [first, last] is the node's run of sorted faces; prefix is the length of
-// the common prefix of its codes (shorter = larger Morton cell).
+// Binary tree node: children >= 0 are nodes, < 0 the leaf ~j (sorted face j; the
+// radix tree only); [first, last] is the node's run of sorted faces.  prefix orders the
+// collapse (smaller opens first): in the radix tree the length of the common prefix of
+// the node's codes (shorter = larger Morton cell), in the SAH tree a key of the node's
+// box area (smaller = larger area).
 struct BinNode {
     int32_t left, right, first, last, parent, prefix;
+};
+
+// Topology of a GPU build.
+enum GpuTopology : int {
+    kTopologyMedian = 0,  // object median on the axis of largest centroid extent
+    kTopologyLinear = 1,  // Morton radix tree (linear BVH)
+    kTopologySah = 2,     // surface-area heuristic over every slot of the sorted lists
 };
 
 struct GpuBuildWorkspace {
@@ -29,7 +38,7 @@ struct GpuBuildWorkspace {
     uint32_t* face_of = nullptr;  // [n] original face of sorted position (= leaf slot)
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    BinNode* bn = nullptr;        // [n-1]
+    BinNode* bn = nullptr;        // [n-1] radix tree; [2n] every node of the SAH tree
     int32_t* leaf_parent = nullptr;
     WideNode* wide = nullptr;     // [n-1] wide nodes (the first `n_nodes` used)
     int4* queue0 = nullptr;
@@ -43,6 +52,19 @@ struct GpuBuildWorkspace {
     void* flag_scan = nullptr;    // [n] scanned lower-half flags of the three lists
     void* flag_tmp = nullptr;
     size_t flag_tmp_bytes = 0;
+    // SAH topology
+    float* fbox = nullptr;               // [6n] face boxes
+    int32_t* rid = nullptr;              // [n] active run (binary node) of each slot, -1 once a leaf
+    unsigned long long* best = nullptr;  // [3][2n] least (cost, split slot) per axis and node
+    uint32_t* split = nullptr;           // [2n] chosen split per node: axis << 28 | slot
+    int32_t* act0 = nullptr;             // [n] active nodes of the level, double-buffered
+    int32_t* act1 = nullptr;
+    int32_t* level_count = nullptr;      // active nodes per level
+    float* box_pre = nullptr;            // [6n] run prefix boxes of one list
+    float* box_suf = nullptr;            // [6n] run suffix boxes of one list, reversed
+    float* grow = nullptr;               // box growth of the area terms
+    void* box_tmp = nullptr;
+    size_t box_tmp_bytes = 0;
     // subtrees above 4,096 faces, reduced in chunks
     int32_t* bigjobs = nullptr;   // job indices, counted in counters[9]
     double* big_part = nullptr;   // chunk partials
@@ -57,17 +79,19 @@ struct GpuBuildResult {
     bool too_deep = false;      // deeper than a walk's stack accepts
 };
 
-size_t gpu_build_workspace_bytes(int64_t n);
-GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n);
+size_t gpu_build_workspace_bytes(int64_t n, int topology);
+GpuBuildWorkspace carve_gpu_build_workspace(void* ws, int64_t n, int topology);
 
 // Steps 1-4 (n > leaf_size faces): wide nodes (without boxes) in w.wide, leaf order in
-// w.face_of.  median: every run of faces is halved at the median centroid on its axis
-// of largest centroid extent (the host's object-median split); otherwise the Morton
-// radix tree (faster to build, slower to query).  res.input_errors is set when an index
+// w.face_of.  kTopologyMedian: every run of faces is halved at the median centroid on
+// its axis of largest centroid extent (the host's object-median split);
+// kTopologyLinear: the Morton radix tree (fastest to build, slowest to query);
+// kTopologySah: every run split where the surface-area cost is least (slowest to build,
+// fastest to query; w must be carved for it).  res.input_errors is set when an index
 // or coordinate is bad (the tree is then meaningless, but every access stayed in
 // bounds); res.too_deep when the tree is deeper than a walk's stack accepts.
-// Synchronizes the stream once.
-hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, bool median,
+// Synchronizes the stream once (the SAH topology a few times more).
+hipError_t gpu_build_tree(const float* V, int64_t nv, const int32_t* F, int64_t n, int32_t leaf_size, int topology,
                           const GpuBuildWorkspace& w, GpuBuildResult& res, hipStream_t s);
 
 // Step 5: triangles and derived arrays into `out`, child boxes into w.wide (the caller

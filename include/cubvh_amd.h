@@ -58,10 +58,14 @@ int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* f
 /* Construction on the GPU (no counterpart in the reference, whose build is the
  * host-only TriangleBvh4::build, src/bvh.cu:515-620): vertices fp32 [n_vertices, 3]
  * and faces int32 [n_faces, 3] in the memory of `device`.  leaf_size: bits 0-7 = faces
- * per leaf, 1..8 (0 = 8); bit 8 clear = object-median splits (every run of faces halved
- * at the median centroid of its longest centroid axis, level by level: the leaves of
- * the host build with bit 8 set), bit 8 set = linear BVH from 63-bit Morton codes
- * (faster to build, slower to query).  Both collapse to the same 4-wide layout.  The
+ * per leaf, 1..8 (0 = 8); bits 8 and 9 clear = object-median splits (every run of faces
+ * halved at the median centroid of its longest centroid axis, level by level: the
+ * leaves of the host build with bit 8 set); bit 9 set = surface-area heuristic (every
+ * run split at the slot of least cost over its three centroid-sorted face lists: the
+ * slowest GPU build, the fastest tree to query; a tree too deep for the walk is
+ * rebuilt with median splits); bit 8 set = linear BVH from 63-bit Morton codes
+ * (fastest to build, slowest to query); both bits set is an error.  All collapse to
+ * the same 4-wide layout.  The
  * tree stays on `device`; nodes and triangles are copied to the host only when a
  * host-side use needs them (export, a forest, another device).  Queries on it return the same results as on a
  * host-built tree.  Face indices and coordinates are checked, and so is the built

@@ -44,7 +44,7 @@ def _leaves_cover_faces(bvh, v, f):
     assert (seen == 1).all()
 
 
-@pytest.mark.parametrize("build", ["gpu", "gpu_lbvh"])
+@pytest.mark.parametrize("build", ["gpu", "gpu_sah", "gpu_lbvh"])
 @pytest.mark.parametrize("leaf", [0, 1, 4])
 def test_gpu_build_structure(gpu, leaf, build):
     v, f, _ = S.perturbed_sphere(5, seed=2)
@@ -79,7 +79,7 @@ def test_gpu_median_build_has_the_host_median_leaves(gpu, leaf):
     assert _leaf_sets(g) == _leaf_sets(h)
 
 
-@pytest.mark.parametrize("build", ["gpu", "gpu_lbvh"])
+@pytest.mark.parametrize("build", ["gpu", "gpu_sah", "gpu_lbvh"])
 def test_gpu_build_queries_equal_host_tree_and_oracle(gpu, oracle, ico4, build):
     v, f = ico4
     p = torch.from_numpy(S.uniform_points(30000, seed=5)).to(gpu)
@@ -104,6 +104,40 @@ def test_gpu_build_queries_equal_host_tree_and_oracle(gpu, oracle, ico4, build):
     b = h.ray_trace(o, dirs)
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+def test_gpu_sah_build_on_duplicate_faces(gpu):
+    """Thousands of copies of one triangle tie every split cost; the SAH levels stop
+    after a bounded number of levels (median splits below), and a tree too deep for
+    the walk's stack is rebuilt with median splits.  Results equal the host tree's."""
+    v, f, _ = S.perturbed_sphere(4, seed=3)
+    f = np.concatenate([f, np.repeat(f[:1], 3000, axis=0)]).astype(f.dtype)
+    g = cubvh_amd.cuBVH(v, f, build="gpu_sah")
+    h = cubvh_amd.cuBVH(v, f)
+    info = g.info()
+    assert info["stack_need"] == 3 * info["depth"] + 1 <= 64
+    _leaves_cover_faces(g, v, f)
+    p = torch.from_numpy(S.uniform_points(20000, seed=9)).to(gpu)
+    a = g.unsigned_distance(p, return_uvw=True)
+    b = h.unsigned_distance(p, return_uvw=True)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_gpu_sah_tree_is_shallower_in_cost_than_median(gpu):
+    """The SAH tree's expected work (sum over nodes of child box area x faces below)
+    is below the median tree's on a non-uniform mesh."""
+    v, f, _ = S.perturbed_sphere(6, seed=4)
+    v = v * np.array([4.0, 1.0, 0.25], np.float32)  # stretched: median splits cut badly
+
+    def sah_cost(bvh):
+        nodes = bvh.state_dict()["nodes"].numpy()
+        b = nodes[:, :24].view(np.float32).reshape(-1, 6, 4)
+        d = np.clip(b[:, 1::2, :] - b[:, 0::2, :], 0, None)
+        area = 2 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+        return float(area.sum())
+
+    assert sah_cost(cubvh_amd.cuBVH(v, f, build="gpu_sah")) < sah_cost(cubvh_amd.cuBVH(v, f, build="gpu"))
 
 
 def test_gpu_build_level6_and_reuse(gpu):
