@@ -446,47 +446,48 @@ __global__ __launch_bounds__(kFinishThreads) void median_finish(int64_t n, int g
 }
 
 // ---- SAH topology
-// Every run of more than leaf_size slots [f, l] is split at the position of least
-// surface-area cost over all three centroid-sorted lists: for the split before slot k
-// of list a, cost = A(faces at [f, k) of list a) * (k - f) + A(faces at [k, l]) * (l + 1 - k),
-// A the surface area of the faces' bounding box grown by `grow` (1% of the scene
-// diagonal, as the host build) on every axis.  The boxes of every prefix and suffix of
-// every run come from segmented scans of the face boxes (by run), one level of the
-// tree per round; the chosen split partitions all three lists stably, as the median
-// topology does.  The binary nodes (every node, leaves included) go to `bn`, which the
-// radix-tree collapse then turns into wide nodes; a node's prefix field holds its own
-// box area made into an ordering key (smaller = larger box), so that the collapse opens
-// the child of largest surface area first, as the host collapse does.  Below level
-// kSahLevels the splits fall back to the median on the axis of largest centroid extent,
-// so every run ends within a bounded number of levels.
+// Every run of more than leaf_size slots [f, l] is split where the surface-area cost
+// A(left faces) * |left| + A(right faces) * |right| is least, A the surface area of
+// the faces' bounding box grown by `grow` (1% of the diagonal of the centroid bounds)
+// on every axis, over three candidate orders: the run's slots of each centroid-sorted
+// list.  A split is a slot k of one list (faces at [f, k) go left), so the stable
+// partition of the three lists is the median topology's.
+//   * runs of at most kSahWaveRun slots: every slot is a candidate; one wave per run
+//     (one to 16 slots per lane) takes the box of every prefix and suffix from
+//     registers and wave scans;
+//   * longer runs: kSahBins centroid bins per list; a bin is a contiguous range of the
+//     sorted list, so each wave reduces its slots by (run, bin) and updates the bin's
+//     count and box with one set of atomics per segment, and one wave per run sweeps
+//     the bins as above.
+// The binary nodes (every node, leaves included) go to `bn`, which the radix-tree
+// collapse then turns into wide nodes; a node's prefix field holds its own box area
+// made into an ordering key (smaller = larger box), so that the collapse opens the
+// child of largest surface area first, as the host collapse does.  Below level
+// kSahLevels the splits fall back to the median on the axis of largest centroid
+// extent, so every run ends within a bounded number of levels.
 constexpr float kSahDilate = 0.01f;
 constexpr int kSahLevels = 40;
 constexpr int kSahMaxLevels = 96;
+constexpr int kSahWaveRun = 1024;
+constexpr int kSahBins = 256;
+constexpr int kSahBinWords = 7;  // count, then the box as order-preserving ints
 
 struct Box6 {
     float lx, ly, lz, hx, hy, hz;
 };
-struct BoxUnion {
-    __host__ __device__ Box6 operator()(const Box6& a, const Box6& b) const {
-        return Box6{fminf(a.lx, b.lx), fminf(a.ly, b.ly), fminf(a.lz, b.lz),
-                    fmaxf(a.hx, b.hx), fmaxf(a.hy, b.hy), fmaxf(a.hz, b.hz)};
-    }
-};
-// Run key and face box of slot i of list `list`, read forward or from the end.
-struct RunAt {
-    const int32_t* rid;
-    int64_t n;
-    bool rev;
-    __host__ __device__ int32_t operator()(int64_t i) const { return rid[rev ? n - 1 - i : i]; }
-};
-struct BoxAt {
-    const uint32_t* list;
-    const Box6* fbox;
-    int64_t n;
-    bool rev;
-    __host__ __device__ Box6 operator()(int64_t i) const { return fbox[list[rev ? n - 1 - i : i]]; }
-};
-
+__device__ __forceinline__ Box6 empty_box() { return Box6{INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}; }
+__device__ __forceinline__ Box6 box_union(const Box6& a, const Box6& b) {
+    return Box6{fminf(a.lx, b.lx), fminf(a.ly, b.ly), fminf(a.lz, b.lz),
+                fmaxf(a.hx, b.hx), fmaxf(a.hy, b.hy), fmaxf(a.hz, b.hz)};
+}
+__device__ __forceinline__ Box6 box_shfl_up(const Box6& b, int off, int width = 64) {
+    return Box6{__shfl_up(b.lx, off, width), __shfl_up(b.ly, off, width), __shfl_up(b.lz, off, width),
+                __shfl_up(b.hx, off, width), __shfl_up(b.hy, off, width), __shfl_up(b.hz, off, width)};
+}
+__device__ __forceinline__ Box6 box_shfl_down(const Box6& b, int off, int width = 64) {
+    return Box6{__shfl_down(b.lx, off, width), __shfl_down(b.ly, off, width), __shfl_down(b.lz, off, width),
+                __shfl_down(b.hx, off, width), __shfl_down(b.hy, off, width), __shfl_down(b.hz, off, width)};
+}
 __device__ __forceinline__ float grown_area(const Box6& b, float g) {
     const float d0 = b.hx - b.lx + g, d1 = b.hy - b.ly + g, d2 = b.hz - b.lz + g;
     return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
@@ -503,94 +504,300 @@ __global__ __launch_bounds__(256) void sah_boxes(int64_t n, const float* __restr
     rid[f] = 0;
 }
 
-// The root node [0, n - 1], active at level 0.  cnt[] is zero on entry.
-__global__ void sah_root(int64_t n, BinNode* bn, int32_t* act, int32_t* cnt, unsigned long long* best, int64_t cap,
-                         int32_t* c) {
+// Per level, 4 words of `stats`: active runs, the longest, the runs longer than
+// kSahWaveRun (bin slot s: the run at active index bigrun[s]), unused.  All zero on
+// entry.  Node ids: the root 0, the children of the run at active index e of a level
+// base + 2e and base + 2e + 1, base = 1 + twice the active runs of the levels above.
+__global__ void sah_root(int64_t n, BinNode* bn, int32_t* act, int32_t* stats, int32_t* bigslot, int32_t* bigrun,
+                         float* grow, int32_t* c) {
     if (threadIdx.x != 0) return;
     bn[0] = BinNode{-1, -1, 0, (int32_t)(n - 1), -1, 0};
     act[0] = 0;
-    cnt[0] = 1;
-    c[40] = 1;
-    for (int a = 0; a < 3; ++a) best[a * cap] = ~0ull;
-}
-
-// grow = kSahDilate * diagonal of the box of all faces (the root run's full prefix).
-__global__ void sah_grow(const Box6* __restrict__ last, float* grow) {
-    if (threadIdx.x != 0) return;
-    const Box6 b = *last;
-    const float dx = b.hx - b.lx, dy = b.hy - b.ly, dz = b.hz - b.lz;
+    stats[0] = 1;
+    stats[1] = (int32_t)n;
+    if (n > kSahWaveRun) {
+        stats[2] = 1;
+        bigslot[0] = 0;
+        bigrun[0] = 0;
+    }
+    const float dx = unordered(c[3]) - unordered(c[0]), dy = unordered(c[4]) - unordered(c[1]),
+                dz = unordered(c[5]) - unordered(c[2]);
     *grow = kSahDilate * sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-// Cost of the split before slot p + 1 of list a, for every slot p of an active run but
-// its last; the least (cost, position) of each run goes to best[run] (one atomic per
-// run and wave: the runs of a wave are contiguous slots).
-__global__ __launch_bounds__(256) void sah_cost(int64_t n, const int32_t* __restrict__ rid,
-                                                const BinNode* __restrict__ bn, const Box6* __restrict__ pre,
-                                                const Box6* __restrict__ sufrev, const float* __restrict__ grow,
-                                                unsigned long long* __restrict__ best) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Records the split of active run r = [f, l] at active index e (axis, first slot of
+// the upper side k, its own box area) and its two children, in every lane with
+// `want` (all 64 lanes of the wave call it); the children longer than leaf go to the
+// next level's list, with one atomic per wave, and the longer than kSahWaveRun get a
+// bin slot.  Past kSahLevels, or without a valid k, the median of the axis of largest
+// centroid extent.
+__device__ __forceinline__ void sah_emit(bool want, int level, int32_t node_base, int32_t e, int32_t r, int32_t f,
+                                         int32_t l, unsigned long long kb, int axis, float area,
+                                         int32_t* __restrict__ stats, int32_t* __restrict__ act_out,
+                                         BinNode* __restrict__ bn, int32_t* __restrict__ bigslot,
+                                         int32_t* __restrict__ bigrun, uint32_t* __restrict__ split, const Lists& L,
+                                         const float* __restrict__ cen, int64_t n, int32_t leaf) {
     const int lane = (int)(threadIdx.x & 63);
-    const int32_t r = p < n ? rid[p] : -1;
-    unsigned long long key = ~0ull;
-    if (r >= 0) {
-        const int32_t f = bn[r].first, l = bn[r].last;
-        if (p < l) {
-            const float g = *grow;
-            const float cost = grown_area(pre[p], g) * (float)(p - f + 1) +
-                               grown_area(sufrev[n - 2 - p], g) * (float)(l - p);
-            key = ((unsigned long long)__float_as_uint(cost) << 32) | (uint32_t)(p + 1);
-        }
-    }
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t klo = __shfl_down((uint32_t)key, off), khi = __shfl_down((uint32_t)(key >> 32), off);
-        const int32_t ro = __shfl_down(r, off);
-        const unsigned long long o = ((unsigned long long)khi << 32) | klo;
-        if (lane + off < 64 && ro == r && o < key) key = o;
-    }
-    const int32_t rprev = __shfl_up(r, 1);
-    if (r >= 0 && (lane == 0 || rprev != r) && key != ~0ull) atomicMin(best + r, key);
-}
-
-// One active run per thread: its split (axis << 28 | slot), its two child nodes, the
-// children still longer than leaf_size to the next level's list.
-__global__ __launch_bounds__(256) void sah_choose(int level, int32_t* __restrict__ cnt, const int32_t* __restrict__ act_in,
-                                                  int32_t* __restrict__ act_out, BinNode* __restrict__ bn,
-                                                  unsigned long long* __restrict__ best, int64_t cap,
-                                                  uint32_t* __restrict__ split, const Box6* __restrict__ pre,
-                                                  const float* __restrict__ grow, Lists L,
-                                                  const float* __restrict__ cen, int64_t n, int32_t leaf, int32_t* c) {
-    const int32_t m = cnt[level];
-    for (int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); e < m; e += (int32_t)(gridDim.x * blockDim.x)) {
-        const int32_t r = act_in[e];
-        const int32_t f = bn[r].first, l = bn[r].last;
-        unsigned long long kb = ~0ull;
-        int axis = 0;
-        for (int a = 0; a < 3; ++a) {
-            const unsigned long long x = best[a * cap + r];
-            if (x < kb) {
-                kb = x;
-                axis = a;
-            }
-        }
-        int32_t k = (int32_t)(uint32_t)kb;
+    int32_t k = f + (int32_t)(uint32_t)kb;  // kb: (cost, left count)
+    const int32_t id = node_base + 2 * e;
+    if (want) {
         if (level >= kSahLevels || kb == ~0ull || k <= f || k > l) {
             axis = run_axis(L.l, cen, n, f, (int64_t)l + 1);
             k = f + (l + 1 - f) / 2;
         }
         split[r] = ((uint32_t)axis << 28) | (uint32_t)k;
-        bn[r].prefix = 0x3fffffff - (int32_t)(__float_as_uint(grown_area(pre[l], *grow)) >> 1);
-        const int32_t id = atomicAdd(&c[40], 2);
+        bn[r].prefix = 0x3fffffff - (int32_t)(__float_as_uint(area) >> 1);
         bn[r].left = id;
         bn[r].right = id + 1;
         bn[id] = BinNode{-1, -1, f, k - 1, r, 0};
         bn[id + 1] = BinNode{-1, -1, k, l, r, 0};
-        const int32_t sz[2] = {k - f, l + 1 - k};
-        for (int h = 0; h < 2; ++h) {
-            if (sz[h] <= leaf) continue;
-            act_out[atomicAdd(&cnt[level + 1], 1)] = id + h;
-            for (int a = 0; a < 3; ++a) best[a * cap + id + h] = ~0ull;
+    }
+    const int32_t s0 = want ? k - f : 0, s1 = want ? l + 1 - k : 0;
+    const bool c0 = s0 > leaf, c1 = s1 > leaf;
+    const uint64_t b0 = __ballot(c0), b1 = __ballot(c1);
+    int32_t* next = stats + 4 * (level + 1);
+    int32_t base = 0;
+    const int32_t total = __builtin_popcountll(b0) + __builtin_popcountll(b1);
+    if (lane == 0 && total > 0) base = atomicAdd(&next[0], total);
+    base = __shfl(base, 0);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int32_t pos = base + __builtin_popcountll(b0 & below) + __builtin_popcountll(b1 & below);
+    if (c0) act_out[pos] = id;
+    if (c1) act_out[pos + (c0 ? 1 : 0)] = id + 1;
+    int32_t longest = max(c0 ? s0 : 0, c1 ? s1 : 0);
+    for (int off = 32; off > 0; off >>= 1) longest = max(longest, __shfl_xor(longest, off));
+    if (lane == 0 && longest > 0) atomicMax(&next[1], longest);
+    if (s0 > kSahWaveRun) {
+        const int32_t slot = atomicAdd(&next[2], 1);
+        bigslot[id] = slot;
+        bigrun[slot] = pos;
+    }
+    if (s1 > kSahWaveRun) {
+        const int32_t slot = atomicAdd(&next[2], 1);
+        bigslot[id + 1] = slot;
+        bigrun[slot] = pos + (c0 ? 1 : 0);
+    }
+}
+
+// Least SAH split of an ordered sequence of `items` items, computed by a group of W
+// lanes (W = 16, 32 or 64; at most W * K items, lane j of the group holds items
+// jK .. jK + K - 1), item i with box and count from load(i, box, count).  The split
+// after item i puts items 0..i left; both sides must hold a face.  Returns (cost bits
+// << 32 | faces on the left) of the least cost (ties to fewer on the left), or ~0;
+// `all` = the union of every box.  The same in every lane of the group.
+template <int K, int W, class Load>
+__device__ __forceinline__ unsigned long long wave_least_split(int items, float g, Load load, Box6& all) {
+    const int lane = (int)(threadIdx.x & (W - 1));
+    const int i0 = lane * K;
+    Box6 sbox[K];
+    int32_t scnt[K];
+    Box6 acc = empty_box();
+    int32_t accn = 0;
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+        if (i0 + i < items) {
+            Box6 b;
+            int32_t m;
+            load(i0 + i, b, m);
+            if (m > 0) acc = box_union(acc, b);
+            accn += m;
         }
+        sbox[i] = acc;
+        scnt[i] = accn;
+    }
+    // boxes and counts of the lanes below and above this one
+    Box6 below = box_shfl_up(acc, 1, W), above = box_shfl_down(acc, 1, W);
+    int32_t nbelow = __shfl_up(accn, 1, W), nabove = __shfl_down(accn, 1, W);
+    if (lane == 0) {
+        below = empty_box();
+        nbelow = 0;
+    }
+    if (lane == W - 1) {
+        above = empty_box();
+        nabove = 0;
+    }
+    for (int off = 1; off < W; off <<= 1) {
+        const Box6 bu = box_shfl_up(below, off, W), ad = box_shfl_down(above, off, W);
+        const int32_t nu = __shfl_up(nbelow, off, W), nd = __shfl_down(nabove, off, W);
+        if (lane >= off) {
+            below = box_union(below, bu);
+            nbelow += nu;
+        }
+        if (lane + off < W) {
+            above = box_union(above, ad);
+            nabove += nd;
+        }
+    }
+    all = box_union(below, box_union(acc, above));
+    Box6 pre = below;
+    int32_t npre = nbelow;
+    unsigned long long key = ~0ull;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        if (i0 + i < items) {
+            Box6 b;
+            int32_t m;
+            load(i0 + i, b, m);
+            if (m > 0) pre = box_union(pre, b);
+            npre += m;
+            const Box6 right = i + 1 < K ? box_union(sbox[i + 1], above) : above;
+            const int32_t nright = (i + 1 < K ? scnt[i + 1] : 0) + nabove;
+            if (npre > 0 && nright > 0) {
+                const float cost = grown_area(pre, g) * (float)npre + grown_area(right, g) * (float)nright;
+                const unsigned long long x = ((unsigned long long)__float_as_uint(cost) << 32) | (uint32_t)npre;
+                if (x < key) key = x;
+            }
+        }
+    }
+    for (int off = W / 2; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)key, off, W), hi = __shfl_xor((uint32_t)(key >> 32), off, W);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        if (o < key) key = o;
+    }
+    return key;
+}
+
+// Runs of LO < size <= W * K slots, one group of W lanes each (W = 16 or 32: several
+// runs per wave): every slot of each list a candidate.  The loop is wave-uniform;
+// groups without a run of their class compute nothing (items = 0).
+template <int K, int W, int LO>
+__global__ __launch_bounds__(256) void sah_wave_runs(int level, int32_t node_base, int32_t* __restrict__ stats,
+                                                     const int32_t* __restrict__ act_in, int32_t* __restrict__ act_out,
+                                                     BinNode* __restrict__ bn, int32_t* __restrict__ bigslot,
+                                                     int32_t* __restrict__ bigrun, uint32_t* __restrict__ split,
+                                                     const Box6* __restrict__ fbox, const float* __restrict__ grow,
+                                                     Lists L, const float* __restrict__ cen, int64_t n, int32_t leaf) {
+    constexpr int kGroups = 64 / W;  // groups per wave
+    const int lane = (int)(threadIdx.x & (W - 1));
+    const int group = (int)((threadIdx.x & 63) / W);
+    const int32_t m = stats[4 * level];
+    const int32_t waves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+    const float g = *grow;
+    for (int32_t e0 = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kGroups; e0 < m;
+         e0 += waves * kGroups) {
+        const int32_t e = e0 + group;
+        int32_t r = 0, f = 0, l = 0;
+        bool valid = e < m;
+        if (valid) {
+            r = act_in[e];
+            f = bn[r].first;
+            l = bn[r].last;
+            valid = l - f + 1 > LO && l - f + 1 <= W * K;
+        }
+        const int32_t items = valid ? l - f + 1 : 0;
+        unsigned long long kb = ~0ull;
+        int axis = 0;
+        Box6 all;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t* list = L.l[a] + f;
+            auto load = [&](int i, Box6& b, int32_t& cnt) {
+                b = fbox[list[i]];
+                cnt = 1;
+            };
+            const unsigned long long x = wave_least_split<K, W>(items, g, load, all);
+            if (x < kb) {
+                kb = x;
+                axis = a;
+            }
+        }
+        sah_emit(valid && lane == 0, level, node_base, e, r, f, l, kb, axis, grown_area(all, g), stats, act_out, bn,
+                 bigslot, bigrun, split, L, cen, n, leaf);
+    }
+}
+
+// Empty bins (count 0, box +inf / -inf) for the level's `nbig` long runs.
+__global__ __launch_bounds__(256) void sah_bin_clear(const int32_t* __restrict__ stats, int level,
+                                                     int32_t* __restrict__ bins) {
+    const int64_t nb = (int64_t)stats[4 * level + 2] * 3 * kSahBins;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t* w = bins + kSahBinWords * i;
+        w[0] = 0;
+        w[1] = w[2] = w[3] = ordered(INFINITY);
+        w[4] = w[5] = w[6] = ordered(-INFINITY);
+    }
+}
+
+// Bins of list a: slot p of a long run falls in bin floor((c - lo) * kSahBins / (hi - lo))
+// of its centroid c, lo and hi the run's first and last centroids along a (the list is
+// sorted), so the bins of a run are contiguous and ascending along the list.
+__global__ __launch_bounds__(256) void sah_bin(int64_t n, int a, const int32_t* __restrict__ rid,
+                                               const BinNode* __restrict__ bn, const int32_t* __restrict__ bigslot,
+                                               const uint32_t* __restrict__ list, const float* __restrict__ cen,
+                                               const Box6* __restrict__ fbox, int32_t* __restrict__ bins) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t r = p < n ? rid[p] : -1;
+    int32_t seg = -1, cnt = 0;
+    Box6 b = empty_box();
+    if (r >= 0) {
+        const int32_t f = bn[r].first, l = bn[r].last;
+        if (l - f + 1 > kSahWaveRun) {
+            const float* ca = cen + a * n;
+            const float lo = ca[list[f]], hi = ca[list[l]], ext = hi - lo;
+            const uint32_t face = list[p];
+            int bin = 0;
+            if (ext > 0.0f) bin = min(max((int)((ca[face] - lo) * ((float)kSahBins / ext)), 0), kSahBins - 1);
+            seg = bigslot[r] * kSahBins + bin;
+            b = fbox[face];
+            cnt = 1;
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        const Box6 o = box_shfl_down(b, off);
+        const int32_t os = __shfl_down(seg, off), on = __shfl_down(cnt, off);
+        if (lane + off < 64 && os == seg) {
+            b = box_union(b, o);
+            cnt += on;
+        }
+    }
+    const int32_t sprev = __shfl_up(seg, 1);
+    if (seg >= 0 && (lane == 0 || sprev != seg)) {
+        int32_t* w = bins + (int64_t)kSahBinWords * (((int64_t)(seg / kSahBins) * 3 + a) * kSahBins + seg % kSahBins);
+        atomicAdd(w, cnt);
+        atomicMin(w + 1, ordered(b.lx));
+        atomicMin(w + 2, ordered(b.ly));
+        atomicMin(w + 3, ordered(b.lz));
+        atomicMax(w + 4, ordered(b.hx));
+        atomicMax(w + 5, ordered(b.hy));
+        atomicMax(w + 6, ordered(b.hz));
+    }
+}
+
+// One long run per wave: the least split over the bins of its three lists.
+__global__ __launch_bounds__(256) void sah_bin_choose(int level, int32_t node_base, int32_t* __restrict__ stats,
+                                                      const int32_t* __restrict__ bins,
+                                                      const int32_t* __restrict__ act_in, int32_t* __restrict__ act_out,
+                                                      BinNode* __restrict__ bn, int32_t* __restrict__ bigslot,
+                                                      int32_t* __restrict__ bigrun, uint32_t* __restrict__ split,
+                                                      const float* __restrict__ grow, Lists L,
+                                                      const float* __restrict__ cen, int64_t n, int32_t leaf) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t nbig = stats[4 * level + 2];
+    const int32_t waves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+    for (int32_t slot = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); slot < nbig; slot += waves) {
+        const int32_t e = bigrun[slot];
+        const int32_t r = act_in[e];
+        const int32_t f = bn[r].first, l = bn[r].last;
+        const float g = *grow;
+        unsigned long long kb = ~0ull;
+        int axis = 0;
+        Box6 all;
+        for (int a = 0; a < 3; ++a) {
+            const int32_t* base = bins + (int64_t)kSahBinWords * (((int64_t)slot * 3 + a) * kSahBins);
+            auto load = [&](int i, Box6& b, int32_t& cnt) {
+                const int32_t* w = base + kSahBinWords * i;
+                cnt = w[0];
+                b = Box6{unordered(w[1]), unordered(w[2]), unordered(w[3]), unordered(w[4]), unordered(w[5]),
+                         unordered(w[6])};
+            };
+            const unsigned long long x = wave_least_split<kSahBins / 64, 64>(kSahBins, g, load, all);
+            if (x < kb) {
+                kb = x;
+                axis = a;
+            }
+        }
+        sah_emit(lane == 0, level, node_base, e, r, f, l, kb, axis, grown_area(all, g), stats, act_out, bn, bigslot,
+                 bigrun, split, L, cen, n, leaf);
     }
 }
 
@@ -1245,16 +1452,7 @@ size_t sort_tmp_bytes(int64_t n) {
 }
 
 using FlagIter = hipcub::TransformInputIterator<gb::U3, gb::LowerFlags, hipcub::CountingInputIterator<int64_t>>;
-using RunIter = hipcub::TransformInputIterator<int32_t, gb::RunAt, hipcub::CountingInputIterator<int64_t>>;
-using BoxIter = hipcub::TransformInputIterator<gb::Box6, gb::BoxAt, hipcub::CountingInputIterator<int64_t>>;
 
-size_t box_scan_bytes(int64_t n) {
-    size_t b = 0;
-    RunIter keys(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{});
-    BoxIter vals(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{});
-    hipcub::DeviceScan::InclusiveScanByKey(nullptr, b, keys, vals, (gb::Box6*)nullptr, gb::BoxUnion(), (int)n);
-    return b;
-}
 
 size_t flag_scan_bytes(int64_t n) {
     size_t b = 0;
@@ -1297,18 +1495,18 @@ size_t lay_out_workspace(void* ws, int64_t n, int topology, GpuBuildWorkspace& w
     w.flag_tmp_bytes = flag_scan_bytes(n);
     w.flag_tmp = take(w.flag_tmp_bytes);
     if (sah) {
+        // runs longer than kSahWaveRun are disjoint: at most n / (kSahWaveRun + 1) + 1
+        const int64_t nbig = n / (gb::kSahWaveRun + 1) + 1;
         w.fbox = (float*)take(6 * n * sizeof(float));
         w.rid = (int32_t*)take(n * sizeof(int32_t));
-        w.best = (unsigned long long*)take(3 * 2 * n * sizeof(unsigned long long));
         w.split = (uint32_t*)take(2 * n * sizeof(uint32_t));
         w.act0 = (int32_t*)take(n * sizeof(int32_t));
         w.act1 = (int32_t*)take(n * sizeof(int32_t));
-        w.level_count = (int32_t*)take((gb::kSahMaxLevels + 1) * sizeof(int32_t));
-        w.box_pre = (float*)take(6 * n * sizeof(float));
-        w.box_suf = (float*)take(6 * n * sizeof(float));
+        w.level_count = (int32_t*)take(4 * (gb::kSahMaxLevels + 1) * sizeof(int32_t));
+        w.bigslot = (int32_t*)take(2 * n * sizeof(int32_t));
+        w.bigrun = (int32_t*)take(nbig * sizeof(int32_t));
+        w.bins = (int32_t*)take(nbig * 3 * gb::kSahBins * gb::kSahBinWords * sizeof(int32_t));
         w.grow = (float*)take(sizeof(float));
-        w.box_tmp_bytes = box_scan_bytes(n);
-        w.box_tmp = take(w.box_tmp_bytes);
     }
     // children above kWaveJobMax faces: disjoint at each depth, so at most
     // kMaxDepth * (n / (kWaveJobMax + 1) + 1) of them
@@ -1385,74 +1583,75 @@ hipError_t median_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w
 }
 
 // SAH topology, steps 2-3: the sorted lists, then one level of binary nodes per round
-// (segmented box scans of each list, the least-cost split of every active run, one
-// stable partition of the three lists) until no run is longer than leaf_size; the
-// binary nodes in w.bn, the first list is the leaf order.  The host reads the count of
-// active runs after the levels a median tree would need and then every 4 levels;
-// finished = false if runs are still active after kSahMaxLevels levels (cannot happen:
-// splits fall back to the median below kSahLevels).
+// until no run is longer than leaf_size: runs longer than kSahWaveRun by bins
+// (sah_bin_clear, sah_bin per list, sah_bin_choose), shorter ones by sah_wave_runs
+// (one launch per size class: one, four or 16 slots per lane); then one stable
+// partition of the three lists.  The binary nodes in w.bn, the first list is the leaf
+// order.  The host reads each level's counts (one synchronization per level) to size
+// its launches.  finished = false if runs are still active after kSahMaxLevels levels
+// (cannot happen: splits fall back to the median below kSahLevels).
 hipError_t sah_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, bool& finished, hipStream_t s) {
     finished = false;
     uint32_t* cur[3] = {w.lists, w.lists + n, w.lists + 2 * n};
     uint32_t* nxt[3] = {w.lists + 3 * n, w.lists + 4 * n, w.lists + 5 * n};
     hipError_t e = sorted_lists(n, w, cur, s);
     if (e != hipSuccess) return e;
-    const int64_t cap = 2 * n;
     gb::Box6* fbox = reinterpret_cast<gb::Box6*>(w.fbox);
-    gb::Box6* pre = reinterpret_cast<gb::Box6*>(w.box_pre);
-    gb::Box6* suf = reinterpret_cast<gb::Box6*>(w.box_suf);
     hipLaunchKernelGGL(gb::sah_boxes, dim3(blocks_for(n)), dim3(256), 0, s, n, w.tri9, fbox, w.rid);
-    hipLaunchKernelGGL(gb::fill_i32, dim3(1), dim3(256), 0, s, w.level_count, (int64_t)gb::kSahMaxLevels + 1, 0);
-    hipLaunchKernelGGL(gb::sah_root, dim3(1), dim3(64), 0, s, n, w.bn, w.act0, w.level_count, w.best, cap,
-                       w.counters);
+    hipLaunchKernelGGL(gb::fill_i32, dim3(2), dim3(256), 0, s, w.level_count, 4 * ((int64_t)gb::kSahMaxLevels + 1), 0);
+    hipLaunchKernelGGL(gb::sah_root, dim3(1), dim3(64), 0, s, n, w.bn, w.act0, w.level_count, w.bigslot, w.bigrun,
+                       w.grow, w.counters);
     int32_t* act_in = w.act0;
     int32_t* act_out = w.act1;
     gb::U3* S = reinterpret_cast<gb::U3*>(w.flag_scan);
-    int halvings = 0;
-    for (int64_t run = n; run > leaf_size; run -= run / 2) ++halvings;
-    int level = 0, batch = halvings + 2;
-    const int64_t max_active = n / (leaf_size + 1) + 1;
-    for (;;) {
-        for (int i = 0; i < batch && level < gb::kSahMaxLevels; ++i, ++level) {
-            const gb::Lists L{{cur[0], cur[1], cur[2]}};
-            for (int a = 0; a < 3; ++a) {
-                size_t tmp = w.box_tmp_bytes;
-                RunIter kf(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{w.rid, n, false});
-                BoxIter vf(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{cur[a], fbox, n, false});
-                e = hipcub::DeviceScan::InclusiveScanByKey(w.box_tmp, tmp, kf, vf, pre, gb::BoxUnion(), (int)n,
-                                                           hipcub::Equality(), s);
-                if (e != hipSuccess) return e;
-                if (level == 0 && a == 0) hipLaunchKernelGGL(gb::sah_grow, dim3(1), dim3(64), 0, s, pre + (n - 1), w.grow);
-                tmp = w.box_tmp_bytes;
-                RunIter kr(hipcub::CountingInputIterator<int64_t>(0), gb::RunAt{w.rid, n, true});
-                BoxIter vr(hipcub::CountingInputIterator<int64_t>(0), gb::BoxAt{cur[a], fbox, n, true});
-                e = hipcub::DeviceScan::InclusiveScanByKey(w.box_tmp, tmp, kr, vr, suf, gb::BoxUnion(), (int)n,
-                                                           hipcub::Equality(), s);
-                if (e != hipSuccess) return e;
-                hipLaunchKernelGGL(gb::sah_cost, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.bn, pre, suf,
-                                   w.grow, w.best + a * cap);
-            }
-            const int64_t active = std::min<int64_t>(level < 30 ? (int64_t)1 << level : max_active, max_active);
-            hipLaunchKernelGGL(gb::sah_choose, dim3(std::min<unsigned>(blocks_for(active), 1024)), dim3(256), 0, s,
-                               level, w.level_count, act_in, act_out, w.bn, w.best, cap, w.split, pre, w.grow, L,
-                               w.cen, n, leaf_size, w.counters);
-            hipLaunchKernelGGL(gb::sah_mark, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.split, L, w.side);
-            FlagIter flags(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{L, w.side});
-            size_t tmp = w.flag_tmp_bytes;
-            e = hipcub::DeviceScan::ExclusiveScan(w.flag_tmp, tmp, flags, S, gb::U3Sum(), gb::U3{0, 0, 0}, (int)n, s);
-            if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(gb::sah_scatter, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.bn, w.split, L,
-                               nxt[0], nxt[1], nxt[2], S, w.side, leaf_size, w.counters);
-            for (int a = 0; a < 3; ++a) std::swap(cur[a], nxt[a]);
-            std::swap(act_in, act_out);
-        }
-        int32_t left = 0;
-        e = hipMemcpyAsync(&left, w.level_count + level, sizeof left, hipMemcpyDeviceToHost, s);
+    int32_t* stats = nullptr;  // pinned: the current level's counts
+    e = hipHostMalloc((void**)&stats, 4 * sizeof(int32_t), hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    struct Pinned {
+        int32_t* p;
+        ~Pinned() { (void)hipHostFree(p); }
+    } pinned{stats};
+    int32_t node_base = 1;  // ids of the children made at this level
+    for (int level = 0;; ++level) {
+        e = hipMemcpyAsync(stats, w.level_count + 4 * level, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
-        if (left == 0) break;
+        const int32_t active = stats[0], longest = stats[1], nbig = stats[2];
+        if (active == 0) break;
         if (level >= gb::kSahMaxLevels) return hipGetLastError();
-        batch = 4;
+        const gb::Lists L{{cur[0], cur[1], cur[2]}};
+        if (nbig > 0) {
+            hipLaunchKernelGGL(gb::sah_bin_clear, dim3(std::min<unsigned>(blocks_for((int64_t)nbig * 3 * gb::kSahBins), 4096)),
+                               dim3(256), 0, s, w.level_count, level, w.bins);
+            for (int a = 0; a < 3; ++a)
+                hipLaunchKernelGGL(gb::sah_bin, dim3(blocks_for(n)), dim3(256), 0, s, n, a, w.rid, w.bn, w.bigslot,
+                                   cur[a], w.cen, fbox, w.bins);
+            hipLaunchKernelGGL(gb::sah_bin_choose, dim3(blocks_for(nbig, 4)), dim3(256), 0, s, level, node_base,
+                               w.level_count, w.bins, act_in, act_out, w.bn, w.bigslot, w.bigrun, w.split, w.grow, L,
+                               w.cen, n, leaf_size);
+        }
+        // short runs by size class: groups of 16, 32 and 64 lanes with one slot each, then
+        // 64 lanes with 4 and 16 slots each
+#define CUBVH_SAH_RUNS(K, W, LO)                                                                                  \
+    hipLaunchKernelGGL((gb::sah_wave_runs<K, W, LO>), dim3(std::min<unsigned>(blocks_for(active, 256 / W), 16384)), \
+                       dim3(256), 0, s, level, node_base, w.level_count, act_in, act_out, w.bn, w.bigslot, w.bigrun, \
+                       w.split, fbox, w.grow, L, w.cen, n, leaf_size)
+        CUBVH_SAH_RUNS(1, 16, 0);
+        if (longest > 16) CUBVH_SAH_RUNS(1, 32, 16);
+        if (longest > 32) CUBVH_SAH_RUNS(1, 64, 32);
+        if (longest > 64) CUBVH_SAH_RUNS(4, 64, 64);
+        if (longest > 256) CUBVH_SAH_RUNS(16, 64, 256);
+#undef CUBVH_SAH_RUNS
+        hipLaunchKernelGGL(gb::sah_mark, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.split, L, w.side);
+        FlagIter flags(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{L, w.side});
+        size_t tmp = w.flag_tmp_bytes;
+        e = hipcub::DeviceScan::ExclusiveScan(w.flag_tmp, tmp, flags, S, gb::U3Sum(), gb::U3{0, 0, 0}, (int)n, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(gb::sah_scatter, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.bn, w.split, L,
+                           nxt[0], nxt[1], nxt[2], S, w.side, leaf_size, w.counters);
+        for (int a = 0; a < 3; ++a) std::swap(cur[a], nxt[a]);
+        std::swap(act_in, act_out);
+        node_base += 2 * active;
     }
     finished = true;
     e = hipMemcpyAsync(w.face_of, cur[0], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);

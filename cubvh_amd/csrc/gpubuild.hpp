@@ -53,18 +53,16 @@ struct GpuBuildWorkspace {
     void* flag_tmp = nullptr;
     size_t flag_tmp_bytes = 0;
     // SAH topology
-    float* fbox = nullptr;               // [6n] face boxes
-    int32_t* rid = nullptr;              // [n] active run (binary node) of each slot, -1 once a leaf
-    unsigned long long* best = nullptr;  // [3][2n] least (cost, split slot) per axis and node
-    uint32_t* split = nullptr;           // [2n] chosen split per node: axis << 28 | slot
-    int32_t* act0 = nullptr;             // [n] active nodes of the level, double-buffered
+    float* fbox = nullptr;          // [6n] face boxes
+    int32_t* rid = nullptr;         // [n] active run (binary node) of each slot, -1 once a leaf
+    uint32_t* split = nullptr;      // [2n] chosen split per node: axis << 28 | slot
+    int32_t* act0 = nullptr;        // [n] active nodes of the level, double-buffered
     int32_t* act1 = nullptr;
-    int32_t* level_count = nullptr;      // active nodes per level
-    float* box_pre = nullptr;            // [6n] run prefix boxes of one list
-    float* box_suf = nullptr;            // [6n] run suffix boxes of one list, reversed
-    float* grow = nullptr;               // box growth of the area terms
-    void* box_tmp = nullptr;
-    size_t box_tmp_bytes = 0;
+    int32_t* level_count = nullptr; // per level: active runs, longest, runs above 1,024 slots
+    int32_t* bigslot = nullptr;     // [2n] bin slot of a run above 1,024 slots
+    int32_t* bigrun = nullptr;      // its node
+    int32_t* bins = nullptr;        // per bin slot, list and bin: count and box
+    float* grow = nullptr;          // box growth of the area terms
     // subtrees above 4,096 faces, reduced in chunks
     int32_t* bigjobs = nullptr;   // job indices, counted in counters[9]
     double* big_part = nullptr;   // chunk partials
