@@ -528,7 +528,8 @@ __global__ void sah_root(int64_t n, BinNode* bn, int32_t* act, int32_t* stats, i
 // Records the split of active run r = [f, l] at active index e (axis, first slot of
 // the upper side k, its own box area) and its two children, in every lane with
 // `want` (all 64 lanes of the wave call it); the children longer than leaf go to the
-// next level's list, with one atomic per wave, and the longer than kSahWaveRun get a
+// next level's list, with one atomic per wave, or per block when `blk` (LDS, 33
+// words) is given and the whole block calls it, and the longer than kSahWaveRun get a
 // bin slot.  Past kSahLevels, or without a valid k, the median of the axis of largest
 // centroid extent.
 __device__ __forceinline__ void sah_emit(bool want, int level, int32_t node_base, int32_t e, int32_t r, int32_t f,
@@ -536,7 +537,8 @@ __device__ __forceinline__ void sah_emit(bool want, int level, int32_t node_base
                                          int32_t* __restrict__ stats, int32_t* __restrict__ act_out,
                                          BinNode* __restrict__ bn, int32_t* __restrict__ bigslot,
                                          int32_t* __restrict__ bigrun, uint32_t* __restrict__ split, const Lists& L,
-                                         const float* __restrict__ cen, int64_t n, int32_t leaf) {
+                                         const float* __restrict__ cen, int64_t n, int32_t leaf,
+                                         int32_t* blk = nullptr) {
     const int lane = (int)(threadIdx.x & 63);
     int32_t k = f + (int32_t)(uint32_t)kb;  // kb: (cost, left count)
     const int32_t id = node_base + 2 * e;
@@ -558,15 +560,38 @@ __device__ __forceinline__ void sah_emit(bool want, int level, int32_t node_base
     int32_t* next = stats + 4 * (level + 1);
     int32_t base = 0;
     const int32_t total = __builtin_popcountll(b0) + __builtin_popcountll(b1);
-    if (lane == 0 && total > 0) base = atomicAdd(&next[0], total);
-    base = __shfl(base, 0);
+    int32_t longest = max(c0 ? s0 : 0, c1 ? s1 : 0);
+    for (int off = 32; off > 0; off >>= 1) longest = max(longest, __shfl_xor(longest, off));
+    if (blk) {
+        const int wave = (int)(threadIdx.x >> 6), nwaves = (int)(blockDim.x >> 6);
+        if (lane == 0) {
+            blk[wave] = total;
+            blk[16 + wave] = longest;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t sum = 0, most = 0;
+            for (int v = 0; v < nwaves; ++v) {
+                const int32_t t = blk[v];
+                blk[v] = sum;
+                sum += t;
+                most = max(most, blk[16 + v]);
+            }
+            blk[32] = sum > 0 ? atomicAdd(&next[0], sum) : 0;
+            if (most > 0) atomicMax(&next[1], most);
+        }
+        __syncthreads();
+        base = blk[32] + blk[wave];
+        __syncthreads();  // blk is reused by the next call
+    } else {
+        if (lane == 0 && total > 0) base = atomicAdd(&next[0], total);
+        base = __shfl(base, 0);
+        if (lane == 0 && longest > 0) atomicMax(&next[1], longest);
+    }
     const uint64_t below = (1ull << lane) - 1ull;
     const int32_t pos = base + __builtin_popcountll(b0 & below) + __builtin_popcountll(b1 & below);
     if (c0) act_out[pos] = id;
     if (c1) act_out[pos + (c0 ? 1 : 0)] = id + 1;
-    int32_t longest = max(c0 ? s0 : 0, c1 ? s1 : 0);
-    for (int off = 32; off > 0; off >>= 1) longest = max(longest, __shfl_xor(longest, off));
-    if (lane == 0 && longest > 0) atomicMax(&next[1], longest);
     if (s0 > kSahWaveRun) {
         const int32_t slot = atomicAdd(&next[2], 1);
         bigslot[id] = slot;
@@ -589,18 +614,23 @@ template <int K, int W, class Load>
 __device__ __forceinline__ unsigned long long wave_least_split(int items, float g, Load load, Box6& all) {
     const int lane = (int)(threadIdx.x & (W - 1));
     const int i0 = lane * K;
-    Box6 sbox[K];
-    int32_t scnt[K];
+    constexpr bool kKeep = K <= 4;  // keep the loaded items for the second pass
+    Box6 sbox[K], ibox[kKeep ? K : 1];
+    int32_t scnt[K], icnt[kKeep ? K : 1];
     Box6 acc = empty_box();
     int32_t accn = 0;
 #pragma unroll
     for (int i = K - 1; i >= 0; --i) {
+        Box6 b = empty_box();
+        int32_t m = 0;
         if (i0 + i < items) {
-            Box6 b;
-            int32_t m;
             load(i0 + i, b, m);
             if (m > 0) acc = box_union(acc, b);
             accn += m;
+        }
+        if constexpr (kKeep) {
+            ibox[i] = b;
+            icnt[i] = m;
         }
         sbox[i] = acc;
         scnt[i] = accn;
@@ -637,7 +667,12 @@ __device__ __forceinline__ unsigned long long wave_least_split(int items, float 
         if (i0 + i < items) {
             Box6 b;
             int32_t m;
-            load(i0 + i, b, m);
+            if constexpr (kKeep) {
+                b = ibox[i];
+                m = icnt[i];
+            } else {
+                load(i0 + i, b, m);
+            }
             if (m > 0) pre = box_union(pre, b);
             npre += m;
             const Box6 right = i + 1 < K ? box_union(sbox[i + 1], above) : above;
@@ -658,24 +693,23 @@ __device__ __forceinline__ unsigned long long wave_least_split(int items, float 
 }
 
 // Runs of LO < size <= W * K slots, one group of W lanes each (W = 16 or 32: several
-// runs per wave): every slot of each list a candidate.  The loop is wave-uniform;
-// groups without a run of their class compute nothing (items = 0).
-template <int K, int W, int LO>
-__global__ __launch_bounds__(256) void sah_wave_runs(int level, int32_t node_base, int32_t* __restrict__ stats,
+// runs per wave): every slot of each list a candidate.  The loop is block-uniform
+// (appends to the next level's list take one atomic per block); groups without a run
+// of their class compute nothing (items = 0).
+template <int K, int W, int LO, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sah_wave_runs(int level, int32_t node_base, int32_t* __restrict__ stats,
                                                      const int32_t* __restrict__ act_in, int32_t* __restrict__ act_out,
                                                      BinNode* __restrict__ bn, int32_t* __restrict__ bigslot,
                                                      int32_t* __restrict__ bigrun, uint32_t* __restrict__ split,
                                                      const Box6* __restrict__ fbox, const float* __restrict__ grow,
                                                      Lists L, const float* __restrict__ cen, int64_t n, int32_t leaf) {
-    constexpr int kGroups = 64 / W;  // groups per wave
+    constexpr int kPerBlock = BLOCK / W;  // runs per block and round
+    __shared__ int32_t blk[33];
     const int lane = (int)(threadIdx.x & (W - 1));
-    const int group = (int)((threadIdx.x & 63) / W);
     const int32_t m = stats[4 * level];
-    const int32_t waves = (int32_t)((gridDim.x * blockDim.x) >> 6);
     const float g = *grow;
-    for (int32_t e0 = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kGroups; e0 < m;
-         e0 += waves * kGroups) {
-        const int32_t e = e0 + group;
+    for (int32_t e0 = (int32_t)blockIdx.x * kPerBlock; e0 < m; e0 += (int32_t)gridDim.x * kPerBlock) {
+        const int32_t e = e0 + (int32_t)(threadIdx.x / W);
         int32_t r = 0, f = 0, l = 0;
         bool valid = e < m;
         if (valid) {
@@ -688,6 +722,7 @@ __global__ __launch_bounds__(256) void sah_wave_runs(int level, int32_t node_bas
         unsigned long long kb = ~0ull;
         int axis = 0;
         Box6 all;
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
             const uint32_t* list = L.l[a] + f;
             auto load = [&](int i, Box6& b, int32_t& cnt) {
@@ -701,7 +736,7 @@ __global__ __launch_bounds__(256) void sah_wave_runs(int level, int32_t node_bas
             }
         }
         sah_emit(valid && lane == 0, level, node_base, e, r, f, l, kb, axis, grown_area(all, g), stats, act_out, bn,
-                 bigslot, bigrun, split, L, cen, n, leaf);
+                 bigslot, bigrun, split, L, cen, n, leaf, blk);
     }
 }
 
@@ -1632,15 +1667,15 @@ hipError_t sah_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, b
         }
         // short runs by size class: groups of 16, 32 and 64 lanes with one slot each, then
         // 64 lanes with 4 and 16 slots each
-#define CUBVH_SAH_RUNS(K, W, LO)                                                                                  \
-    hipLaunchKernelGGL((gb::sah_wave_runs<K, W, LO>), dim3(std::min<unsigned>(blocks_for(active, 256 / W), 16384)), \
-                       dim3(256), 0, s, level, node_base, w.level_count, act_in, act_out, w.bn, w.bigslot, w.bigrun, \
+#define CUBVH_SAH_RUNS(K, W, LO, B)                                                                               \
+    hipLaunchKernelGGL((gb::sah_wave_runs<K, W, LO, B>), dim3(std::min<unsigned>(blocks_for(active, B / W), 16384)),  \
+                       dim3(B), 0, s, level, node_base, w.level_count, act_in, act_out, w.bn, w.bigslot, w.bigrun,   \
                        w.split, fbox, w.grow, L, w.cen, n, leaf_size)
-        CUBVH_SAH_RUNS(1, 16, 0);
-        if (longest > 16) CUBVH_SAH_RUNS(1, 32, 16);
-        if (longest > 32) CUBVH_SAH_RUNS(1, 64, 32);
-        if (longest > 64) CUBVH_SAH_RUNS(4, 64, 64);
-        if (longest > 256) CUBVH_SAH_RUNS(16, 64, 256);
+        CUBVH_SAH_RUNS(1, 16, 0, 1024);
+        if (longest > 16) CUBVH_SAH_RUNS(1, 32, 16, 1024);
+        if (longest > 32) CUBVH_SAH_RUNS(1, 64, 32, 1024);
+        if (longest > 64) CUBVH_SAH_RUNS(4, 64, 64, 1024);
+        if (longest > 256) CUBVH_SAH_RUNS(16, 64, 256, 256);
 #undef CUBVH_SAH_RUNS
         hipLaunchKernelGGL(gb::sah_mark, dim3(blocks_for(n)), dim3(256), 0, s, n, w.rid, w.split, L, w.side);
         FlagIter flags(hipcub::CountingInputIterator<int64_t>(0), gb::LowerFlags{L, w.side});
