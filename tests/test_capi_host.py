@@ -274,6 +274,21 @@ def test_build_options_and_depth_fallback():
     assert info["stack_need"] <= 64
 
 
+def test_gpu_build_flags_checked_before_device_work():
+    """cubvh_bvh_create_gpu: leaf_size bits 8 (linear BVH) and 9 (SAH) exclude each
+    other, and higher bits are unknown; both are refused before any device access."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    for flags, msg in ((0x300, b"exclude"), (0x400, b"unknown bits"), (0x209, b"0..8")):
+        assert lib.cubvh_bvh_create_gpu(0, 1, 3, 1, 100, flags, ctypes.byref(h), None) == -1
+        assert msg in lib.cubvh_last_error(), lib.cubvh_last_error()
+        assert not h.value
+
+
 def test_ordered_entry_points_validate_arguments(ico4):
     """Argument errors are reported before any device work."""
     import cubvh_amd

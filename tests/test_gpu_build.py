@@ -124,6 +124,28 @@ def test_gpu_sah_build_on_duplicate_faces(gpu):
         assert torch.equal(x, y)
 
 
+def test_gpu_sah_build_on_geometric_clusters(gpu):
+    """Triangles at geometrically shrinking scales make very unbalanced SAH splits; the
+    GPU SAH build still yields a tree the walk accepts (median splits past 40 levels,
+    or the median rebuild), with the host tree's results."""
+    tris = []
+    for k in range(3000):
+        s = 0.7 ** (k % 60)
+        c = np.array([s, 0.0, 0.0])
+        tris.append([c, c + [s * 0.1, 0, 0], c + [0, s * 0.1, 0]])
+    v = np.asarray(tris, np.float32).reshape(-1, 3)
+    f = np.arange(len(v), dtype=np.int32).reshape(-1, 3)
+    g = cubvh_amd.cuBVH(v, f, build="gpu_sah")
+    assert g.info()["stack_need"] <= 64
+    _leaves_cover_faces(g, v, f)
+    h = cubvh_amd.cuBVH(v, f)
+    p = torch.from_numpy(S.uniform_points(20000, seed=11)).to(gpu)
+    a = g.unsigned_distance(p, return_uvw=True)
+    b = h.unsigned_distance(p, return_uvw=True)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
 def test_gpu_sah_tree_is_shallower_in_cost_than_median(gpu):
     """The SAH tree's expected work (sum over nodes of child box area x faces below)
     is below the median tree's on a non-uniform mesh."""
