@@ -224,14 +224,10 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
     mesh_dev = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f in meshes]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    gbvhs = []
-    for v, f in mesh_dev:
-        h = ctypes.c_void_p()
-        cubvh_amd._lib.check(lib.cubvh_bvh_create_gpu(dev.index, v.data_ptr(), v.shape[0], f.data_ptr(), f.shape[0],
-                                                      0, ctypes.byref(h), torch.cuda.current_stream(dev).cuda_stream))
-        gbvhs.append(h)
+    # four builds in flight (one host thread and stream each)
+    gbvhs = cubvh_amd.cuBVH.build_many(mesh_dev, build="gpu", device=dev, streams=4)
     gforest = ctypes.c_void_p()
-    arr = (ctypes.c_void_p * n_meshes)(*[h.value for h in gbvhs])
+    arr = (ctypes.c_void_p * n_meshes)(*[b._h.value for b in gbvhs])
     cubvh_amd._lib.check(lib.cubvh_forest_create(arr, n_meshes, ctypes.byref(gforest)))
     gpu_build_s = time.perf_counter() - t0
     stream = torch.cuda.current_stream(dev).cuda_stream
@@ -244,8 +240,7 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
 
     _, t_gforest = _time_kernel(gpu_forest, steps, warmup)
     lib.cubvh_forest_destroy(gforest)
-    for h in gbvhs:
-        lib.cubvh_bvh_destroy(h)
+    del gbvhs
     return {"workload": f"{n_meshes} perturbed icospheres, {int(forest.info()['n_faces'])} faces, "
                         f"{n} uniform points in [-1,1]^3 (4,096..262,144 per mesh)",
             "queries": int(n), "faces": int(forest.info()["n_faces"]), "mesh_generation_s": gen_s,

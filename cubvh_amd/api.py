@@ -90,6 +90,58 @@ class cuBVH:
         self._faces = f
         self._vertices = v
 
+    @staticmethod
+    def build_many(meshes, build: str = "gpu", leaf_size: int = 0, device=None, streams: int = 4):
+        """Trees for many meshes built on the GPU concurrently: `streams` host threads,
+        each with its own stream (a GPU build is a few dozen short launches and one or
+        more synchronizations, so one mesh alone leaves the GPU mostly idle).  meshes:
+        (vertices, triangles) pairs as accepted by the constructor (host arrays are
+        copied to `device`; device tensors are read where they are, after the current
+        stream's pending work).  build: "gpu", "gpu_sah" or "gpu_lbvh".  Returns the
+        cuBVH objects in input order, as `cuBVH(v, f, build=build)` would."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        flags = {"gpu": 0, "gpu_lbvh": 0x100, "gpu_sah": 0x200}.get(build)
+        if flags is None:
+            raise ValueError(f"build must be 'gpu', 'gpu_sah' or 'gpu_lbvh', not {build!r}")
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        lib = _lib.load()
+        producer = torch.cuda.current_stream(dev)
+        pool = [torch.cuda.Stream(dev) for _ in range(max(1, int(streams)))]
+        for st in pool:
+            st.wait_stream(producer)
+
+        def one(job):
+            i, (vertices, triangles) = job
+            st = pool[i % len(pool)]
+            with torch.cuda.device(dev), torch.cuda.stream(st):
+                if torch.is_tensor(vertices) and vertices.device == dev and torch.is_tensor(triangles) \
+                        and triangles.device == dev:
+                    vt = vertices.detach().to(torch.float32).contiguous()
+                    ft = triangles.detach().to(torch.int32).contiguous()
+                    v = f = None
+                else:
+                    v, f = _as_mesh_arrays(vertices, triangles)
+                    vt = torch.from_numpy(v).to(dev)
+                    ft = torch.from_numpy(f).to(dev)
+                assert ft.shape[0] > 8, "BVH needs at least 8 triangles."
+                h = ctypes.c_void_p()
+                _lib.check(lib.cubvh_bvh_create_gpu(dev.index, vt.data_ptr(), vt.shape[0], ft.data_ptr(), ft.shape[0],
+                                                    int(leaf_size) | flags, ctypes.byref(h),
+                                                    ctypes.c_void_p(st.cuda_stream)))
+                obj = cuBVH.__new__(cuBVH)
+                obj._h = h
+                obj._lib = lib
+                if v is None:
+                    v = vt.cpu().numpy()
+                    f = ft.cpu().numpy()
+                obj._vertices = v
+                obj._faces = f
+                return obj
+
+        with ThreadPoolExecutor(max_workers=len(pool)) as ex:
+            return list(ex.map(one, enumerate(meshes)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

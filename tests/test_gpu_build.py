@@ -183,6 +183,24 @@ def test_gpu_build_level6_and_reuse(gpu):
     assert torch.equal(f2[: p.shape[0]], fh) and torch.equal(f2[p.shape[0]:], fh)
 
 
+@pytest.mark.parametrize("build", ["gpu", "gpu_sah"])
+def test_build_many_equals_one_by_one(gpu, build):
+    """cuBVH.build_many: concurrent builds on 4 streams, host and device inputs; each
+    tree answers as the tree built alone."""
+    meshes = [S.perturbed_sphere(3 + (i % 3), seed=20 + i)[:2] for i in range(9)]
+    inputs = [(v, f) if i % 2 else (torch.from_numpy(v).to(gpu), torch.from_numpy(f).to(gpu))
+              for i, (v, f) in enumerate(meshes)]
+    many = cubvh_amd.cuBVH.build_many(inputs, build=build, device=gpu, streams=4)
+    p = torch.from_numpy(S.uniform_points(5000, seed=12)).to(gpu)
+    for (v, f), b in zip(meshes, many):
+        one = cubvh_amd.cuBVH(v, f, build=build)
+        assert b.info() == one.info()
+        for x, y in zip(b.unsigned_distance(p, return_uvw=True), one.unsigned_distance(p, return_uvw=True)):
+            assert torch.equal(x, y)
+    forest = cubvh_amd.cuBVHForest(many)
+    assert forest.info()["n_faces"] == sum(len(f) for _, f in meshes)
+
+
 def test_gpu_build_checks_input(gpu):
     from cubvh_amd import _lib
 
