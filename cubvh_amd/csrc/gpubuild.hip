@@ -1639,13 +1639,7 @@ hipError_t sah_order(int64_t n, int32_t leaf_size, const GpuBuildWorkspace& w, b
     int32_t* act_in = w.act0;
     int32_t* act_out = w.act1;
     gb::U3* S = reinterpret_cast<gb::U3*>(w.flag_scan);
-    int32_t* stats = nullptr;  // pinned: the current level's counts
-    e = hipHostMalloc((void**)&stats, 4 * sizeof(int32_t), hipHostMallocDefault);
-    if (e != hipSuccess) return e;
-    struct Pinned {
-        int32_t* p;
-        ~Pinned() { (void)hipHostFree(p); }
-    } pinned{stats};
+    int32_t stats[4];  // the current level's counts (the copy completes before the synchronization)
     int32_t node_base = 1;  // ids of the children made at this level
     for (int level = 0;; ++level) {
         e = hipMemcpyAsync(stats, w.level_count + 4 * level, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s);

@@ -247,7 +247,8 @@ class RefBvh:
         return self
 
     def __del__(self):
-        if getattr(self, "h", None):
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "h", None) and load is not None:
             load().orc_refbvh_free(self.h)
             self.h = None
 

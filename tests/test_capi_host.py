@@ -274,6 +274,30 @@ def test_build_options_and_depth_fallback():
     assert info["stack_need"] <= 64
 
 
+def test_host_trees_reference_work(oracle):
+    """Tree quality as the reference's walk counts it: the host SAH tree, exported to
+    the reference layout and walked by the oracle's restated reference traversal
+    (src/bvh.cu:300-353), takes fewer box and triangle tests per query than the
+    reference's own build (src/bvh.cu:515-620) on a level-6 sphere; the object-median
+    tree does not beat the SAH tree; every tree gives the same distances."""
+    import cubvh_amd
+    from cubvh_amd import synthetic as S
+
+    v, f, _ = S.perturbed_sphere(6, seed=5)
+    pts = S.uniform_points(1 << 14, seed=13)
+    ref = oracle.RefBvh(v, f)
+    d_ref, _, _, c = ref.udf(pts, return_uvw=False, tie_mode=0)
+    cost = {"reference": 24 * c[0] + 36 * c[1]}
+    for name, flags in (("sah", 0), ("median", 0x100)):
+        sd = cubvh_amd.cuBVH(v, f, leaf_size=flags).state_dict(layout="reference")
+        tree = oracle.RefBvh.from_arrays(sd["nodes"].numpy(), sd["triangles"].numpy())
+        d, _, _, c = tree.udf(pts, return_uvw=False, tie_mode=0)
+        np.testing.assert_array_equal(d, d_ref)
+        cost[name] = 24 * c[0] + 36 * c[1]
+    assert cost["sah"] < cost["reference"], cost
+    assert cost["sah"] <= cost["median"], cost
+
+
 def test_gpu_build_flags_checked_before_device_work():
     """cubvh_bvh_create_gpu: leaf_size bits 8 (linear BVH) and 9 (SAH) exclude each
     other, and higher bits are unknown; both are refused before any device access."""

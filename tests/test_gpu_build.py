@@ -201,6 +201,24 @@ def test_build_many_equals_one_by_one(gpu, build):
     assert forest.info()["n_faces"] == sum(len(f) for _, f in meshes)
 
 
+def test_sah_trees_do_less_reference_work_than_the_reference_tree(gpu, oracle):
+    """Tree quality as the reference's walk counts it (N_box, N_tri per query): the
+    host and GPU SAH trees, exported to the reference layout and walked by the
+    oracle's restated reference traversal, cost fewer bytes per query than the
+    reference's own build; the distances equal."""
+    v, f, _ = S.perturbed_sphere(6, seed=5)
+    pts = S.uniform_points(1 << 14, seed=13)
+    ref = oracle.RefBvh(v, f)
+    d_ref, _, _, c = ref.udf(pts, return_uvw=False, tie_mode=0)
+    ref_bytes = 24 * c[0] + 36 * c[1]
+    for kw in ({}, {"build": "gpu_sah"}):
+        sd = cubvh_amd.cuBVH(v, f, **kw).state_dict(layout="reference")
+        tree = oracle.RefBvh.from_arrays(sd["nodes"].numpy(), sd["triangles"].numpy())
+        d, _, _, c = tree.udf(pts, return_uvw=False, tie_mode=0)
+        np.testing.assert_array_equal(d, d_ref)
+        assert 24 * c[0] + 36 * c[1] < ref_bytes, (kw, c, ref_bytes)
+
+
 def test_gpu_build_checks_input(gpu):
     from cubvh_amd import _lib
 
