@@ -451,8 +451,9 @@ def main():
                     help="run only the sparse marching cubes leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path has no "
-              f"multi-GPU form in this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path runs on one "
+              f"GPU; queries would shard across GPUs with a replicated tree, which is not implemented)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
