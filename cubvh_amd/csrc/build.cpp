@@ -219,8 +219,8 @@ inline float pad_up(float v, float abs_pad) {
 std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, int leaf_size, HostBvh& out,
                       int split_mode) {
     if (n_faces <= 0) return "mesh has no faces";
-    if (n_faces > MAX_LEAF_FIRST) return "too many faces for the 28-bit leaf encoding";
-    if (leaf_size < 1 || leaf_size > MAX_LEAF_SIZE) return "leaf_size must be in 1..8";
+    if (n_faces > MAX_LEAF_FIRST) return "too many faces for the 27-bit leaf encoding";
+    if (leaf_size < 1 || leaf_size > MAX_LEAF_SIZE) return "leaf_size must be in 1..16";
 
     Builder B;
     B.tri9 = tri9;
@@ -506,8 +506,8 @@ void compute_face_bounds(HostBvh& bvh) {
         for (int k = 0; k < 4; ++k) {
             const int32_t ref = w.child[k];
             if (ref == CHILD_EMPTY || ref >= 0) continue;
-            const int64_t first = (int64_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
-            const int count = (ref & 7) + 1;
+            const int64_t first = leaf_ref_first(ref);
+            const int count = leaf_ref_count(ref);
             if (first + count > m) continue;  // rejected by validate_bvh
             double v[3 * MAX_LEAF_SIZE][3], n[3] = {0.0, 0.0, 0.0};
             double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -553,8 +553,8 @@ void compute_face_bounds(HostBvh& bvh) {
                             if (bvh.nodes[ref].child[c] != CHILD_EMPTY) st.push_back(bvh.nodes[ref].child[c]);
                         continue;
                     }
-                    const int64_t first = (int64_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
-                    const int count = (ref & 7) + 1;
+                    const int64_t first = leaf_ref_first(ref);
+                    const int count = leaf_ref_count(ref);
                     for (int j = 0; j < count && first + j < m; ++j) {
                         double v[3][3], fn[3];
                         face_vertices(bvh, first + j, v);
@@ -606,8 +606,8 @@ std::string validate_bvh(HostBvh& bvh) {
             if (ref >= 0) {
                 st.push_back({ref, d + 1});
             } else {
-                int64_t first = (int64_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
-                int cnt = (int)(ref & 7) + 1;
+                int64_t first = leaf_ref_first(ref);
+                int cnt = leaf_ref_count(ref);
                 if (first + cnt > bvh.n_faces) return "leaf range out of bounds";
                 for (int j = 0; j < cnt; ++j) {
                     if (covered[first + j]) return "triangle in two leaves";

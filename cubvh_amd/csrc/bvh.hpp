@@ -30,8 +30,8 @@
 // Child references (WideNode::child[k]):
 //   ref >= 0                  internal node index
 //   ref == CHILD_EMPTY        unused slot (node with fewer than 4 children)
-//   otherwise (bit 31 set)    leaf: first triangle = (ref & 0x7fffffff) >> 3,
-//                             count = (ref & 7) + 1
+//   otherwise (bit 31 set)    leaf: first triangle = (ref & 0x7fffffff) >> 4,
+//                             count = (ref & 15) + 1 (leaf_ref_first / leaf_ref_count)
 #pragma once
 
 #include <cstdint>
@@ -42,12 +42,16 @@ namespace cubvh_amd {
 
 constexpr int32_t CHILD_EMPTY = -1;
 constexpr int32_t LEAF_BIT = (int32_t)0x80000000u;
-constexpr int64_t MAX_LEAF_FIRST = (1ll << 28) - 2;  // keeps every leaf ref != CHILD_EMPTY
-constexpr int MAX_LEAF_SIZE = 8;
+constexpr int LEAF_COUNT_BITS = 4;
+constexpr int64_t MAX_LEAF_FIRST = (1ll << (31 - LEAF_COUNT_BITS)) - 2;  // keeps every leaf ref != CHILD_EMPTY
+constexpr int MAX_LEAF_SIZE = 1 << LEAF_COUNT_BITS;
 
-inline int32_t make_leaf_ref(int64_t first, int count) {
-    return (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(count - 1));
+// constexpr: usable in host and device code
+constexpr int32_t make_leaf_ref(int64_t first, int count) {
+    return (int32_t)(0x80000000u | ((uint32_t)first << LEAF_COUNT_BITS) | (uint32_t)(count - 1));
 }
+constexpr int32_t leaf_ref_first(int32_t ref) { return (int32_t)(((uint32_t)ref & 0x7fffffffu) >> LEAF_COUNT_BITS); }
+constexpr int32_t leaf_ref_count(int32_t ref) { return (ref & (MAX_LEAF_SIZE - 1)) + 1; }
 
 struct alignas(16) WideNode {
     float lo_x[4], hi_x[4];

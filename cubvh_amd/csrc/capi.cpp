@@ -35,7 +35,7 @@ int fail_hip(hipError_t e, const char* what) {
 }
 
 constexpr int kMaxDevices = 64;
-constexpr int kDefaultLeaf = 8;  // faces per leaf unless the caller asks otherwise
+constexpr int kDefaultLeaf = 16;  // faces per leaf unless the caller asks otherwise
 // Point-query batches at least this large are sorted into Morton order and walked as
 // wave packets; smaller ones are walked one lane per query in input order.
 constexpr int64_t kMortonMin = 1 << 11;
@@ -440,7 +440,7 @@ int cubvh_bvh_create_gpu(int device, const float* vertices, int64_t n_vertices, 
     // bit 8: Morton radix tree, bit 9: surface-area heuristic, neither: object median
     const int topology = (leaf_size & 0x100) ? kTopologyLinear : (leaf_size & 0x200) ? kTopologySah : kTopologyMedian;
     leaf_size &= 0xff;
-    if (leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..8");
+    if (leaf_size > MAX_LEAF_SIZE) return fail(CUBVH_ERR_ARG, "leaf_size must be in 0..16");
     if (leaf_size == 0) leaf_size = kDefaultLeaf;
     if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
     hipStream_t hs = (hipStream_t)stream;
@@ -579,8 +579,8 @@ int cubvh_bvh_export_reference(const cubvh_bvh* bvh, int32_t* nodes9, int64_t* n
             r.mn[0] = w.lo_x[q]; r.mn[1] = w.lo_y[q]; r.mn[2] = w.lo_z[q];
             r.mx[0] = w.hi_x[q]; r.mx[1] = w.hi_y[q]; r.mx[2] = w.hi_z[q];
             if (ref < 0) {
-                const int32_t first = (int32_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
-                const int32_t count = (ref & 7) + 1;
+                const int32_t first = leaf_ref_first(ref);
+                const int32_t count = leaf_ref_count(ref);
                 r.left = -first - 1;
                 r.right = -(first + count) - 1;
             } else {
@@ -647,7 +647,7 @@ int cubvh_bvh_import(const int32_t* nodes, int64_t n_nodes, const int32_t* trian
     }
     for (const WideNode& w : h.nodes)
         for (int k = 0; k < 4; ++k)
-            if (w.child[k] != CHILD_EMPTY && w.child[k] < 0) leaf_max = std::max<int64_t>(leaf_max, (w.child[k] & 7) + 1);
+            if (w.child[k] != CHILD_EMPTY && w.child[k] < 0) leaf_max = std::max<int64_t>(leaf_max, leaf_ref_count(w.child[k]));
     h.leaf_size = (int32_t)leaf_max;
     std::string err = validate_bvh(h);
     if (!err.empty()) {
@@ -864,7 +864,7 @@ int forest_create_on_device(cubvh_bvh* const* bvhs, int32_t n_meshes, int32_t de
         const cubvh_bvh* b = bvhs[m];
         if (f->n_faces + b->host.n_faces > MAX_LEAF_FIRST || f->n_nodes + b->n_nodes > 0x7fffffff) {
             delete f;
-            return fail(CUBVH_ERR_ARG, "forest too large for the 28-bit leaf encoding");
+            return fail(CUBVH_ERR_ARG, "forest too large for the 27-bit leaf encoding");
         }
         f->mesh_info[4 * m + 0] = (int32_t)f->n_nodes;
         f->mesh_info[4 * m + 1] = (int32_t)(f->n_faces + b->host.face0_slot);
@@ -942,7 +942,7 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
         const int64_t tri_base = f->n_faces;
         if (tri_base + h.n_faces > MAX_LEAF_FIRST || node_base + (int64_t)h.nodes.size() > 0x7fffffff) {
             delete f;
-            return fail(CUBVH_ERR_ARG, "forest too large for the 28-bit leaf encoding");
+            return fail(CUBVH_ERR_ARG, "forest too large for the 27-bit leaf encoding");
         }
         for (WideNode w : h.nodes) {
             for (int k = 0; k < 4; ++k) {
@@ -951,8 +951,7 @@ int cubvh_forest_create(cubvh_bvh* const* bvhs, int32_t n_meshes, cubvh_forest**
                 if (r >= 0) {
                     w.child[k] = (int32_t)(r + node_base);
                 } else {
-                    int64_t first = (int64_t)(((uint32_t)r & 0x7fffffffu) >> 3);
-                    w.child[k] = make_leaf_ref(first + tri_base, (r & 7) + 1);
+                    w.child[k] = make_leaf_ref((int64_t)leaf_ref_first(r) + tri_base, leaf_ref_count(r));
                 }
             }
             f->nodes.push_back(w);

@@ -177,8 +177,7 @@ __global__ __launch_bounds__(256) void rebase_nodes(WideNode* nodes, int64_t n, 
         if (r >= 0) {
             w.child[k] = r + node_base;
         } else {
-            const uint32_t first = (((uint32_t)r & 0x7fffffffu) >> 3) + (uint32_t)tri_base;
-            w.child[k] = (int32_t)(0x80000000u | (first << 3) | ((uint32_t)r & 7u));
+            w.child[k] = make_leaf_ref((int64_t)leaf_ref_first(r) + tri_base, leaf_ref_count(r));
         }
     }
 }

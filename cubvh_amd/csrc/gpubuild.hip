@@ -981,7 +981,7 @@ __device__ __forceinline__ void collapse_entry(const int4* __restrict__ in, int3
         const int32_t first = r < 0 ? ~r : bn[r].first;
         const int32_t cnt = size_of(r);
         if (cnt <= leaf_size) {
-            w.child[q] = (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(cnt - 1));  // leaf reference
+            w.child[q] = make_leaf_ref(first, cnt);
             leaves[b_leaf++] = make_int4(first, cnt, ent.y, q);
         } else {
             const int32_t id = b_wide++;
@@ -1058,7 +1058,7 @@ __device__ __forceinline__ void collapse_run_entry(const int4* __restrict__ in, 
     for (int q = 0; q < nk; ++q) {
         const int32_t first = kb[q], cnt = ke[q] - kb[q];
         if (cnt <= leaf_size) {
-            w.child[q] = (int32_t)(0x80000000u | ((uint32_t)first << 3) | (uint32_t)(cnt - 1));  // leaf reference
+            w.child[q] = make_leaf_ref(first, cnt);
             leaves[b_leaf++] = make_int4(first, cnt, ent.w, q);
         } else {
             const int32_t id = b_wide++;

@@ -79,8 +79,8 @@ __device__ __forceinline__ void load_tri(const float4* __restrict__ tris, int32_
 }
 
 __device__ __forceinline__ void leaf_range(int32_t ref, int32_t& first, int32_t& count) {
-    first = (int32_t)(((uint32_t)ref & 0x7fffffffu) >> 3);
-    count = (ref & 7) + 1;
+    first = leaf_ref_first(ref);
+    count = leaf_ref_count(ref);
 }
 
 // Ascending 4-element sort network on (key, ref).

@@ -48,7 +48,8 @@ int cubvh_abi_version(void);
  *          TriangleBvh4::build                 src/bvh.cu:515-620.
  * vertices: fp32 [n_vertices, 3] row-major; faces: int32 [n_faces, 3] row-major.
  * Face indices are bounds-checked (the reference does not check them).
- * leaf_size: bits 0-7 = maximum triangles per leaf, 1..8 (0 selects the default, 8);
+ * leaf_size: bits 0-7 = maximum triangles per leaf, 1..16 (0 selects the default, 16: distance
+ * queries run fastest on it; leaves of 8 suit ray tracing better);
  * bit 8 set = split at the object median instead of the binned surface-area
  * heuristic (the default).
  * ------------------------------------------------------------------------- */
@@ -58,7 +59,7 @@ int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* f
 /* Construction on the GPU (no counterpart in the reference, whose build is the
  * host-only TriangleBvh4::build, src/bvh.cu:515-620): vertices fp32 [n_vertices, 3]
  * and faces int32 [n_faces, 3] in the memory of `device`.  leaf_size: bits 0-7 = faces
- * per leaf, 1..8 (0 = 8); bits 8 and 9 clear = object-median splits (every run of faces
+ * per leaf, 1..16 (0 = 16); bits 8 and 9 clear = object-median splits (every run of faces
  * halved at the median centroid of its longest centroid axis, level by level: the
  * leaves of the host build with bit 8 set); bit 9 set = surface-area heuristic (every
  * run split at the slot of least cost over its three centroid-sorted face lists: the

@@ -1,7 +1,8 @@
 """Tree quality as the reference's own walk sees it: N_box and N_tri per query.
 
 Every tree this library builds (host SAH and host median, GPU median, GPU SAH, GPU
-linear BVH) is exported to the reference's state_dict layout and walked by the
+linear BVH), at the reference's leaf size (at most 8 faces) and, for the host SAH
+tree, at this library's default (16), is exported to the reference's state_dict layout and walked by the
 oracle's instrumented restatement of the reference UDF traversal (src/bvh.cu:300-353,
 oracle.RefBvh: 4 box tests per internal node popped, every triangle of every leaf
 popped), beside the reference's own build (src/bvh.cu:515-620, restated in the
@@ -50,10 +51,12 @@ def main():
     out["trees"]["reference build (4-ary median, leaves <= 8)"] = measure(ref, pts)
     print("reference", out["trees"]["reference build (4-ary median, leaves <= 8)"], f"{time.time() - t0:.1f}s",
           flush=True)
-    for name, kw in (("host SAH (default)", {}), ("host object median", {"leaf_size": 0x100}),
-                     ("GPU object median (build='gpu')", {"build": "gpu"}),
-                     ("GPU SAH (build='gpu_sah')", {"build": "gpu_sah"}),
-                     ("GPU linear BVH (build='gpu_lbvh')", {"build": "gpu_lbvh"})):
+    for name, kw in (("host SAH, leaves <= 8", {"leaf_size": 8}),
+                     ("host object median, leaves <= 8", {"leaf_size": 0x108}),
+                     ("GPU object median (build='gpu'), leaves <= 8", {"build": "gpu", "leaf_size": 8}),
+                     ("GPU SAH (build='gpu_sah'), leaves <= 8", {"build": "gpu_sah", "leaf_size": 8}),
+                     ("GPU linear BVH (build='gpu_lbvh'), leaves <= 8", {"build": "gpu_lbvh", "leaf_size": 8}),
+                     ("host SAH, leaves <= 16 (default)", {})):
         b = cubvh_amd.cuBVH(v, f, **kw)
         sd = b.state_dict(layout="reference")
         tree = O.RefBvh.from_arrays(sd["nodes"].numpy(), sd["triangles"].numpy())

@@ -173,7 +173,7 @@ def test_tree_properties_large_mesh():
             if r == -1 or r >= 0:
                 continue
             u = np.uint32(r & 0x7FFFFFFF)
-            first, cnt = int(u >> 3), int(u & 7) + 1
+            first, cnt = int(u >> 4), int(u & 15) + 1
             seen[first:first + cnt] += 1
             tri = abc[first:first + cnt]
             lo = bounds[ni, 0::2, k]
@@ -259,7 +259,7 @@ def test_build_options_and_depth_fallback():
     from cubvh_amd import synthetic as S
 
     v, f = S.icosphere(4)
-    for flags, leaf in ((0, 8), (4, 4), (0x104, 4), (0x101, 1)):
+    for flags, leaf in ((0, 16), (8, 8), (4, 4), (0x104, 4), (0x101, 1), (16, 16)):
         info = cubvh_amd.cuBVH(v, f, leaf_size=flags).info()
         assert info["leaf_size"] <= leaf and info["stack_need"] <= 64
     # triangles at geometrically shrinking scales: very unbalanced SAH splits
@@ -275,11 +275,12 @@ def test_build_options_and_depth_fallback():
 
 
 def test_host_trees_reference_work(oracle):
-    """Tree quality as the reference's walk counts it: the host SAH tree, exported to
-    the reference layout and walked by the oracle's restated reference traversal
-    (src/bvh.cu:300-353), takes fewer box and triangle tests per query than the
-    reference's own build (src/bvh.cu:515-620) on a level-6 sphere; the object-median
-    tree does not beat the SAH tree; every tree gives the same distances."""
+    """Tree quality as the reference's walk counts it: at the reference's leaf size (8)
+    the host SAH tree, exported to the reference layout and walked by the oracle's
+    restated reference traversal (src/bvh.cu:300-353), takes fewer box and triangle
+    tests per query than the reference's own build (src/bvh.cu:515-620) on a level-6
+    sphere; the object-median tree does not beat the SAH tree; every tree gives the
+    same distances."""
     import cubvh_amd
     from cubvh_amd import synthetic as S
 
@@ -288,7 +289,7 @@ def test_host_trees_reference_work(oracle):
     ref = oracle.RefBvh(v, f)
     d_ref, _, _, c = ref.udf(pts, return_uvw=False, tie_mode=0)
     cost = {"reference": 24 * c[0] + 36 * c[1]}
-    for name, flags in (("sah", 0), ("median", 0x100)):
+    for name, flags in (("sah", 8), ("median", 0x108)):
         sd = cubvh_amd.cuBVH(v, f, leaf_size=flags).state_dict(layout="reference")
         tree = oracle.RefBvh.from_arrays(sd["nodes"].numpy(), sd["triangles"].numpy())
         d, _, _, c = tree.udf(pts, return_uvw=False, tie_mode=0)
@@ -307,7 +308,7 @@ def test_gpu_build_flags_checked_before_device_work():
 
     lib = _lib.load()
     h = ctypes.c_void_p()
-    for flags, msg in ((0x300, b"exclude"), (0x400, b"unknown bits"), (0x209, b"0..8")):
+    for flags, msg in ((0x300, b"exclude"), (0x400, b"unknown bits"), (0x211, b"0..16")):
         assert lib.cubvh_bvh_create_gpu(0, 1, 3, 1, 100, flags, ctypes.byref(h), None) == -1
         assert msg in lib.cubvh_last_error(), lib.cubvh_last_error()
         assert not h.value
