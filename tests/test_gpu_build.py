@@ -187,7 +187,8 @@ def test_gpu_build_level6_and_reuse(gpu):
 def test_build_many_equals_one_by_one(gpu, build):
     """cuBVH.build_many: concurrent builds on 4 streams, host and device inputs; each
     tree answers as the tree built alone."""
-    meshes = [S.perturbed_sphere(3 + (i % 3), seed=20 + i)[:2] for i in range(9)]
+    # 320 faces (one short run: the lane-group path from the root), 1,280 and 5,120 (bins at the top)
+    meshes = [S.perturbed_sphere(2 + (i % 3), seed=20 + i)[:2] for i in range(9)]
     inputs = [(v, f) if i % 2 else (torch.from_numpy(v).to(gpu), torch.from_numpy(f).to(gpu))
               for i, (v, f) in enumerate(meshes)]
     many = cubvh_amd.cuBVH.build_many(inputs, build=build, device=gpu, streams=4)
