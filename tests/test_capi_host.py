@@ -299,6 +299,15 @@ def test_host_trees_reference_work(oracle):
     assert cost["sah"] <= cost["median"], cost
 
 
+def test_build_many_rejects_unknown_build():
+    import cubvh_amd
+
+    with pytest.raises(ValueError, match="gpu_sah"):
+        cubvh_amd.cuBVH.build_many([], build="host")
+    with pytest.raises(ValueError, match="gpu_lbvh"):
+        cubvh_amd.cuBVH([[0, 0, 0]] * 30, [[0, 1, 2]] * 10, build="cpu")
+
+
 def test_gpu_build_flags_checked_before_device_work():
     """cubvh_bvh_create_gpu: leaf_size bits 8 (linear BVH) and 9 (SAH) exclude each
     other, and higher bits are unknown; both are refused before any device access."""
