@@ -45,7 +45,7 @@ def _leaves_cover_faces(bvh, v, f):
 
 
 @pytest.mark.parametrize("build", ["gpu", "gpu_sah", "gpu_lbvh"])
-@pytest.mark.parametrize("leaf", [0, 1, 4])
+@pytest.mark.parametrize("leaf", [0, 1, 4, 13, 16])
 def test_gpu_build_structure(gpu, leaf, build):
     v, f, _ = S.perturbed_sphere(5, seed=2)
     bvh = cubvh_amd.cuBVH(v, f, leaf_size=leaf, build=build)
@@ -219,6 +219,26 @@ def test_sah_trees_do_less_reference_work_than_the_reference_tree(gpu, oracle):
         d, _, _, c = tree.udf(pts, return_uvw=False, tie_mode=0)
         np.testing.assert_array_equal(d, d_ref)
         assert 24 * c[0] + 36 * c[1] < ref_bytes, (kw, c, ref_bytes)
+
+
+def test_results_do_not_depend_on_leaf_size(gpu, ico4):
+    """Every query form gives the same answers on host trees with leaves of 1, 8, 13 and
+    16 faces (conservative boxes and disks, ties to the lowest face id)."""
+    v, f = ico4
+    p = torch.from_numpy(S.uniform_points(20000, seed=21)).to(gpu)
+    o, dirs = (torch.from_numpy(x).to(gpu) for x in S.sphere_rays(20000, seed=22))
+    ref = None
+    for leaf in (1, 8, 13, 16):
+        b = cubvh_amd.cuBVH(v, f, leaf_size=leaf)
+        out = list(b.unsigned_distance(p, return_uvw=True))
+        for mode in ("watertight", "raystab"):
+            out += list(b.signed_distance(p[:8192], mode=mode, return_uvw=True))
+        out += list(b.ray_trace(o, dirs))
+        if ref is None:
+            ref = out
+        else:
+            for x, y in zip(out, ref):
+                assert torch.equal(x, y), leaf
 
 
 def test_gpu_build_checks_input(gpu):
