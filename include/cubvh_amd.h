@@ -51,7 +51,8 @@ int cubvh_abi_version(void);
  * leaf_size: bits 0-7 = maximum triangles per leaf, 1..16 (0 selects the default, 16: distance
  * queries run fastest on it; leaves of 8 suit ray tracing better);
  * bit 8 set = split at the object median instead of the binned surface-area
- * heuristic (the default).
+ * heuristic (the default).  n_faces <= 2^27 - 2 (a leaf reference holds a 27-bit
+ * first-face index and a 4-bit count); a forest's meshes together likewise.
  * ------------------------------------------------------------------------- */
 int cubvh_bvh_create(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
                      int32_t leaf_size, cubvh_bvh** out);
