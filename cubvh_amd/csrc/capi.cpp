@@ -988,6 +988,10 @@ int cubvh_forest_info(const cubvh_forest* forest, int64_t info[4]) {
     return CUBVH_OK;
 }
 
+// Largest Morton-bin count of a forest call (n_meshes << order bits): bin indices are
+// 32-bit and the bin histogram takes 4 bytes per bin.
+constexpr int64_t kMaxForestBins = (int64_t)1 << 30;
+
 int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, int32_t max_order_bits,
                          void* stream) {
     CUBVH_GUARD_BEGIN
@@ -996,6 +1000,8 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
     if (max_queries < 0 || max_queries >= ((int64_t)1 << 32) - 64 * (int64_t)forest->n_meshes)
         return fail(CUBVH_ERR_ARG, "max_queries out of range");
     if (max_order_bits < 0 || max_order_bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
+    if (((int64_t)forest->n_meshes << max_order_bits) > kMaxForestBins)
+        return fail(CUBVH_ERR_ARG, "n_meshes * 2^order_bits exceeds 2^30 Morton bins: use fewer order bits");
     std::lock_guard<std::mutex> lock(forest->mu);
     DeviceScope g(device);
     if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
@@ -1019,6 +1025,9 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     const int32_t persistent = tuning ? tuning[1] : 1;
     const int32_t wpc = tuning ? tuning[2] : 8;
     if (bits < 0 || bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
+    // a bin index is mesh << bits | code in 32 bits; the histogram holds one word per bin
+    if (((int64_t)forest->n_meshes << bits) > kMaxForestBins)
+        return fail(CUBVH_ERR_ARG, "n_meshes * 2^order_bits exceeds 2^30 Morton bins: use fewer order bits");
     if (wpc < 1 || wpc > 32) return fail(CUBVH_ERR_ARG, "waves per CU must be in 1..32");
     hipStream_t hs = (hipStream_t)stream;
     std::lock_guard<std::mutex> lock(forest->mu);

@@ -48,7 +48,8 @@ __global__ void padded_bases(const int64_t* __restrict__ off, int32_t m, int64_t
     const int t = threadIdx.x;
     const int per = (m + 1023) / 1024;
     int64_t s = 0;
-    for (int k = t * per; k < min(m, (t + 1) * per); ++k) s += ((off[k + 1] - off[k] + 63) / 64) * 64;
+    // a negative count (offsets that decrease) counts as an empty mesh
+    for (int k = t * per; k < min(m, (t + 1) * per); ++k) s += ((max(off[k + 1] - off[k], (int64_t)0) + 63) / 64) * 64;
     part[t] = s;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
@@ -60,7 +61,7 @@ __global__ void padded_bases(const int64_t* __restrict__ off, int32_t m, int64_t
     int64_t base = t > 0 ? part[t - 1] : 0;
     for (int k = t * per; k < min(m, (t + 1) * per); ++k) {
         pbase[k] = base;
-        base += ((off[k + 1] - off[k] + 63) / 64) * 64;
+        base += ((max(off[k + 1] - off[k], (int64_t)0) + 63) / 64) * 64;
     }
     if (t == 1023) pbase[m] = part[1023];
 }
@@ -141,12 +142,16 @@ __global__ __launch_bounds__(1024) void scan_sums(uint32_t* __restrict__ sums, i
     }
 }
 
+// Slots outside [0, npad_max) are skipped: with offsets that are not a CSR of n
+// queries (offsets[0] != 0, decreasing, offsets[M] != n) the answers are wrong but
+// every write stays inside the order array.
 __global__ __launch_bounds__(256) void scatter_order(int64_t n, const int64_t* __restrict__ off, int32_t m,
                                                      int32_t bits, const uint32_t* __restrict__ scan,
                                                      const uint32_t* __restrict__ sums,
                                                      const int64_t* __restrict__ pbase,
                                                      const uint32_t* __restrict__ bin_of,
-                                                     const uint32_t* __restrict__ rank, uint32_t* __restrict__ order) {
+                                                     const uint32_t* __restrict__ rank, uint32_t* __restrict__ order,
+                                                     int64_t npad_max) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t bin = bin_of[i];
@@ -154,7 +159,8 @@ __global__ __launch_bounds__(256) void scatter_order(int64_t n, const int64_t* _
     const uint32_t first = mesh << bits;
     const uint32_t pos_bin = scan[bin] + sums[bin >> 10];
     const uint32_t pos_first = scan[first] + sums[first >> 10];
-    order[pbase[mesh] + (int64_t)(pos_bin - pos_first) + rank[i]] = (uint32_t)i;
+    const int64_t slot = pbase[mesh] + (int64_t)(pos_bin - pos_first) + rank[i];
+    if (slot >= 0 && slot < npad_max) order[slot] = (uint32_t)i;
 }
 
 __global__ void reset_counter(uint32_t* c) {
@@ -244,7 +250,7 @@ hipError_t forest_order(int64_t n, const float* positions, const int64_t* offset
     hipLaunchKernelGGL(dev::scan_sums, dim3(1), dim3(1024), 0, s, w.sums, (int32_t)w.nchunks);
     if (n > 0)
         hipLaunchKernelGGL(dev::scatter_order, dim3(g256), dim3(256), 0, s, n, offsets, n_meshes, bits, w.hist, w.sums,
-                           w.pbase, w.bin_of, w.rank, w.order);
+                           w.pbase, w.bin_of, w.rank, w.order, w.npad_max);
     hipLaunchKernelGGL(dev::reset_counter, dim3(1), dim3(64), 0, s, w.counter);
     return hipGetLastError();
 }

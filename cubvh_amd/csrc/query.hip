@@ -934,10 +934,13 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
                                                               const int4* __restrict__ mesh_info,
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
-                                                              TreeArrays arrays, uint32_t* counter) {
+                                                              TreeArrays arrays, uint32_t* counter,
+                                                              int64_t max_tiles) {
     __shared__ ExactQueue queues[4];
     const int lane = (int)(threadIdx.x & 63);
-    const int64_t n_tiles = pbase[n_meshes] / 64;
+    // max_tiles: the tiles the order array holds (bounds the walk even when the offsets
+    // are not a CSR of the queries)
+    const int64_t n_tiles = min(pbase[n_meshes] / 64, max_tiles);
     int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
     for (;;) {
         if (PERSISTENT) {
@@ -1105,12 +1108,14 @@ hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws
     if (a.persistent) {
         const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter);
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter,
+                           ws.npad_max / 64);
     } else {
         const int64_t max_tiles = ws.npad_max / 64;
         const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
         hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter);
+                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter,
+                           max_tiles);
     }
     return hipGetLastError();
 }

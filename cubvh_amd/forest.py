@@ -22,13 +22,26 @@ from .api import cuBVH, _query_tensor, _stream_of
 DEFAULT_TUNING = (12, 1, 8)  # Morton-prefix bits, persistent grid, waves per CU
 
 
-def _offsets_tensor(offsets, n, device):
+def _offsets_tensor(offsets, n, device, check=True):
+    """offsets as an int64 tensor on `device`.  check: verify on the host that they are
+    a CSR of the n queries (offsets[0] == 0, non-decreasing, offsets[-1] == n); the
+    kernels read them on the device only.  Offsets already on a GPU are copied back for
+    the check (a synchronization), so callers capturing a graph check before capture."""
     if not torch.is_tensor(offsets):
         offsets = torch.as_tensor(offsets)
-    offsets = offsets.to(device=device, dtype=torch.int64).contiguous()
     if offsets.dim() != 1 or offsets.numel() < 2:
         raise ValueError("offsets must be a 1-D tensor of n_meshes + 1 entries")
-    return offsets
+    if offsets.dtype.is_floating_point or offsets.dtype == torch.bool:
+        raise ValueError(f"offsets must be integers, got {offsets.dtype}")
+    if check:
+        h = offsets.detach().to("cpu", torch.int64)
+        if int(h[0]) != 0:
+            raise ValueError(f"offsets[0] must be 0, got {int(h[0])}")
+        if int(h[-1]) != n:
+            raise ValueError(f"offsets[-1] must equal the number of queries {n}, got {int(h[-1])}")
+        if bool((h[1:] < h[:-1]).any()):
+            raise ValueError("offsets must be non-decreasing")
+    return offsets.to(device=device, dtype=torch.int64).contiguous()
 
 
 class cuBVHForest:
@@ -65,15 +78,17 @@ class cuBVHForest:
         s = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         _lib.check(self._lib.cubvh_forest_reserve(self._h, dev.index, int(max_queries), int(max_order_bits), s))
 
-    def unsigned_distance(self, points, offsets, return_uvw=False, tuning=None, out=None):
+    def unsigned_distance(self, points, offsets, return_uvw=False, tuning=None, out=None, check=True):
         """Distances of points [N, 3] (concatenated per mesh, split by offsets [M + 1])
         to their own mesh.  Returns (distances [N], face_id [N] int64, uvw [N, 3] or None).
-        ``out`` may pass preallocated (distances, face_id, uvw) buffers."""
+        ``out`` may pass preallocated (distances, face_id, uvw) buffers.  check=False
+        skips the host check of the offsets (for calls captured in a graph, whose
+        offsets were checked before capture)."""
         points = _query_tensor(points)
         points = points.view(-1, 3)
         n = points.shape[0]
         dev = points.device
-        offsets = _offsets_tensor(offsets, n, dev)
+        offsets = _offsets_tensor(offsets, n, dev, check)
         if offsets.numel() != self.n_meshes + 1:
             raise ValueError(f"offsets must have {self.n_meshes + 1} entries")
         if out is None:
@@ -127,11 +142,11 @@ class ForestGraph:
         s.wait_stream(torch.cuda.current_stream(dev))
         with torch.cuda.stream(s):
             for _ in range(warmup):
-                forest.unsigned_distance(self.points, self.offsets, return_uvw, t, out)
+                forest.unsigned_distance(self.points, self.offsets, return_uvw, t, out, check=False)
         torch.cuda.current_stream(dev).wait_stream(s)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            forest.unsigned_distance(self.points, self.offsets, return_uvw, t, out)
+            forest.unsigned_distance(self.points, self.offsets, return_uvw, t, out, check=False)
 
     def replay(self):
         self.graph.replay()
@@ -141,7 +156,7 @@ class ForestGraph:
 class _BatchedUDF(torch.autograd.Function):
     @staticmethod
     def forward(ctx, points, vertices, forest, offsets, faces, face_base_per_query, tuning):
-        dist, face, uvw = forest.unsigned_distance(points, offsets, True, tuning)
+        dist, face, uvw = forest.unsigned_distance(points, offsets, True, tuning, check=False)
         gface = face + face_base_per_query
         ctx.save_for_backward(points, gface, uvw, vertices, faces)
         ctx.mark_non_differentiable(face, uvw)

@@ -198,7 +198,8 @@ void cubvh_forest_destroy(cubvh_forest* forest);
 int cubvh_forest_info(const cubvh_forest* forest, int64_t info[4]);
 
 /* Uploads the forest to `device` and allocates the query workspace for up to
- * max_queries queries and Morton-prefix bins of up to max_order_bits bits.  After
+ * max_queries queries and Morton-prefix bins of up to max_order_bits bits
+ * (n_meshes * 2^max_order_bits <= 2^30).  After
  * this call, cubvh_forest_unsigned_distance within those limits allocates nothing
  * and issues only kernel launches (safe to capture in a hipGraph).  Synchronous. */
 int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, int32_t max_order_bits,
@@ -206,9 +207,12 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
 
 /* Unsigned distance of n queries, positions fp32 [n,3] concatenated mesh by mesh;
  * query_offsets: int64 [n_meshes + 1] ON THE DEVICE, queries of mesh m are
- * [query_offsets[m], query_offsets[m+1]).  Outputs as cubvh_unsigned_distance.
+ * [query_offsets[m], query_offsets[m+1]).  The offsets must be a CSR of the n queries
+ * (offsets[0] = 0, non-decreasing, offsets[n_meshes] = n); they are read on the device
+ * only, so other offsets are not reported: they give wrong answers, but every write
+ * stays inside the outputs and the workspace.  Outputs as cubvh_unsigned_distance.
  * tuning (nullable = {12, 1, 8}): [0] Morton-prefix bits used to order the queries
- * inside each mesh (0..16, 0 keeps input order), [1] 1 = persistent grid taking
+ * inside each mesh (0..16, 0 keeps input order; n_meshes * 2^bits <= 2^30), [1] 1 = persistent grid taking
  * 64-query tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of
  * the persistent grid (1..32).  Grows the workspace when the reserved one is too
  * small, which is refused while the stream is being captured. */

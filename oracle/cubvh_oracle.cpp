@@ -1203,6 +1203,94 @@ int orc_refbvh_sdf(void* h, const float* pts, int64_t n, uint32_t mode, float* d
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Canonical watertight normal at c through the reference tree: every face with
+// tri_dist2(c) < 1e-6 (children pruned with the padded box, which bounds every face's
+// fp32 squared distance from below), unit normals summed in double, then rounded and
+// divided by the count -- the same sum as brute_avg_normal over all faces.
+V3 ref_avg_normal_canon(const RefBvh& b, V3 c) {
+    const float EPS = 1e-6f;
+    std::vector<int> st{0};
+    double sx = 0, sy = 0, sz = 0;
+    float cnt = 0;
+    while (!st.empty()) {
+        const RefNode& nd = b.nodes[st.back()];
+        st.pop_back();
+        if (nd.left_idx < 0) {
+            int end = -nd.right_idx - 1;
+            for (int i = -nd.left_idx - 1; i < end; ++i)
+                if (tri_dist2(b.tris[i], c) < EPS) {
+                    V3 n = tri_normal(b.tris[i]);
+                    sx += n.x;
+                    sy += n.y;
+                    sz += n.z;
+                    cnt += 1.0f;
+                }
+        } else {
+            for (int k = 0; k < 4; ++k)
+                if (box_dist2_padded(b.nodes[nd.left_idx + k].bb, c) < EPS) st.push_back(nd.left_idx + k);
+        }
+    }
+    return v3((float)sx / cnt, (float)sy / cnt, (float)sz / cnt);
+}
+}  // namespace
+
+extern "C" {
+
+// Signed distance through the reference tree for queries that carry their raystab RNG
+// index explicitly: qidx[i] is the index query i has within the caller's whole call
+// (src/bvh.cu:663-668 keys the generator by it), so a sample of a large call can be
+// checked on its own.  qidx NULL means 0..n-1.
+// rule 0: the reference's own rules (last-visited distance ties, fp32 normal sum in
+//         visit order, safe_divide intersector and slab test), as orc_refbvh_sdf;
+// rule 1: canonical, the product's bit-exact target (distance ties to the lowest face
+//         id, face 0 when nothing is within 1000, normal sum in double over every face
+//         within squared distance 1e-6, exact-division intersector, conservative slabs).
+int orc_refbvh_sdf_idx(void* h, const float* pts, const int64_t* qidx, int64_t n, uint32_t mode, int rule,
+                       float* dist, int64_t* face, float* uvw, int nthreads) {
+    const RefBvh& b = *(RefBvh*)h;
+    const int tm = rule ? 1 : 0;
+    int face0 = 0;
+    if (tm == 1)
+        for (size_t k = 0; k < b.tris.size(); ++k)
+            if (b.tris[k].id == 0) { face0 = (int)k; break; }
+    int nt = set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t i = 0; i < n; ++i) {
+        V3 p = load3(pts, i);
+        bool miss = false;
+        auto r = ref_closest(b, p, MAX_DIST_SQ, nullptr, tm, &miss);
+        int ti = (miss && tm == 1) ? face0 : r.first;
+        const Tri& t = b.tris[ti];
+        float d = r.second;
+        if (mode == 0) {
+            V3 cp = tri_closest_pt(t, p);
+            V3 nrm = tm ? ref_avg_normal_canon(b, cp) : ref_avg_normal(b, cp, nullptr);
+            d = std::copysign(d, dot(nrm, sub(p, cp)));
+        } else {
+            const int64_t q = qidx ? qidx[i] : i;
+            Pcg32 rng;
+            rng.advance((int64_t)(uint32_t)((uint32_t)q * 2u));
+            float ox = rng.next_float(), oy = rng.next_float();
+            bool inside = true;
+            for (uint32_t k = 0; k < 32; ++k) {
+                V3 dd = fibonacci_dir32(k, ox, oy);
+                if (ref_ray(b, p, neg(dd), nullptr, tm).first < 0 || ref_ray(b, p, dd, nullptr, tm).first < 0) {
+                    inside = false;
+                    break;
+                }
+            }
+            if (inside) d = -d;
+        }
+        dist[i] = d;
+        face[i] = t.id;
+        epilogue_uvw(t, p, uvw, i);
+    }
+    return 0;
+}
+
 // ---- floodfill ----------------------------------------------------------------
 // Restatement of include/gpu/floodfill.cuh:157-272 for one batch (labels global
 // across the batch).  Neighbour decoding follows the reference (x = local % W,

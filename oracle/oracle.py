@@ -21,21 +21,23 @@ _lib = None
 REFERENCE = Path("/root/reference")
 REF_HASH_LIB = HERE / "_ref" / "libref_hashtable.so"
 REF_SMC_LIB = HERE / "_ref" / "libref_smc.so"
+REF_PCG_LIB = HERE / "_ref" / "libref_pcg32.so"
 
 
 def build(verbose: bool = False) -> None:
     """Compile both oracle variants with the committed Makefile (g++), and the
-    reference's own CPU hash table into _ref/ when the reference sources are present."""
+    reference's own CPU hash table, CPU marching cubes and PCG32 generator into _ref/
+    when the reference sources are present."""
     r = subprocess.run(["make", "-B", "-C", str(HERE)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout)
     if verbose:
         print(r.stdout)
-    if (REFERENCE / "include" / "cpu" / "hashtable.h").exists() and (REFERENCE / "include" / "cpu" / "spmc.h").exists():
+    if all((REFERENCE / "include" / h).exists() for h in ("cpu/hashtable.h", "cpu/spmc.h", "gpu/pcg32.h")):
         r = subprocess.run(["make", "-B", "-C", str(HERE), "ref", f"REF={REFERENCE}"], stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, text=True)
         if r.returncode != 0:
-            raise RuntimeError("reference CPU hash table / marching cubes build failed:\n" + r.stdout)
+            raise RuntimeError("reference CPU hash table / marching cubes / PCG32 build failed:\n" + r.stdout)
 
 
 def _cpu_has_fma() -> bool:
@@ -82,6 +84,7 @@ def load():
         "orc_refbvh_udf": (c_int, [vp, vp, i64, vp, vp, vp, c_int, vp, c_int]),
         "orc_refbvh_ray": (c_int, [vp, vp, vp, i64, vp, vp, vp, c_int, vp, c_int]),
         "orc_refbvh_sdf": (c_int, [vp, vp, i64, c_uint32, vp, vp, vp, vp, c_int]),
+        "orc_refbvh_sdf_idx": (c_int, [vp, vp, vp, i64, c_uint32, c_int, vp, vp, vp, c_int]),
         "orc_floodfill_ref": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
         "orc_floodfill_bfs": (c_int, [vp, c_int, c_int, c_int, c_int, vp]),
         "orc_hash_build": (i64, [vp, i64, c_int, c_int, vp]),
@@ -294,6 +297,22 @@ class RefBvh:
         load().orc_refbvh_sdf(self.h, _p(pts), n, int(mode), _p(d), _p(f), _p(uvw), _p(counts), nthreads)
         return d, f, uvw, counts
 
+    def sdf_idx(self, pts, mode, qidx=None, rule=1, return_uvw=True, nthreads=0):
+        """Signed distance of pts (mode 0 watertight, else raystab) where query i has
+        raystab RNG index qidx[i] within the caller's whole call (None: 0..n-1).
+        rule 1: canonical (the product's bit-exact target); rule 0: the reference's
+        own rules (safe_divide intersector, last-visited ties, fp32 normal sum)."""
+        pts = _f32(pts).reshape(-1, 3)
+        n = len(pts)
+        q = None if qidx is None else np.ascontiguousarray(qidx, dtype=np.int64)
+        if q is not None and q.shape != (n,):
+            raise ValueError("qidx must have one entry per point")
+        d = np.empty(n, np.float32)
+        f = np.empty(n, np.int64)
+        uvw = np.empty((n, 3), np.float32) if return_uvw else None
+        load().orc_refbvh_sdf_idx(self.h, _p(pts), _p(q), n, int(mode), int(rule), _p(d), _p(f), _p(uvw), nthreads)
+        return d, f, uvw
+
 
 # ---------------------------------------------------------------- floodfill
 def floodfill_ref(grid):
@@ -433,3 +452,45 @@ def ref_smc_cpu(coords, corners, iso, consistent=False):
     _ref_smc_lib().ref_smc(_p(c), _p(v), n, float(iso), int(bool(consistent)), _p(verts), len(verts), _p(tris),
                            len(tris), ctypes.byref(nv), ctypes.byref(nt))
     return verts[: nv.value].copy(), tris[: nt.value].copy()
+
+
+# ---------------------------------------------------------------- reference PCG32
+_ref_pcg = None
+
+
+def ref_pcg32_available() -> bool:
+    return REF_PCG_LIB.exists()
+
+
+def _ref_pcg_lib():
+    global _ref_pcg
+    if _ref_pcg is None:
+        lib = ctypes.CDLL(str(REF_PCG_LIB))
+        lib.ref_pcg32_uints.restype = None
+        lib.ref_pcg32_uints.argtypes = [c_int64, c_int64, c_void_p]
+        lib.ref_pcg32_offsets.restype = None
+        lib.ref_pcg32_offsets.argtypes = [c_int64, c_void_p]
+        lib.ref_pcg32_state.restype = ctypes.c_uint64
+        lib.ref_pcg32_state.argtypes = [c_int64]
+        _ref_pcg = lib
+    return _ref_pcg
+
+
+def ref_pcg32_uints(advance, n):
+    """next_uint() outputs of the reference's own generator (include/gpu/pcg32.h, built
+    into _ref/), default state, advanced by `advance` first."""
+    out = np.empty(n, np.uint32)
+    _ref_pcg_lib().ref_pcg32_uints(int(advance), int(n), _p(out))
+    return out
+
+
+def ref_pcg32_offsets(n):
+    """The reference generator's raystab offsets of queries 0..n-1: float32 [n, 2]."""
+    out = np.empty((n, 2), np.float32)
+    _ref_pcg_lib().ref_pcg32_offsets(int(n), _p(out))
+    return out
+
+
+def ref_pcg32_state(advance):
+    """The reference generator's internal 64-bit state after advance(advance)."""
+    return int(_ref_pcg_lib().ref_pcg32_state(int(advance)))

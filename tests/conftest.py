@@ -44,3 +44,23 @@ def gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def record():
+    """record(name, **figures): prints the measured figures of a parity test and, when
+    CUBVH_PARITY_LOG names a file, appends them to it as one JSON line."""
+    import json
+    import os
+
+    path = os.environ.get("CUBVH_PARITY_LOG")
+
+    def _record(name, **figures):
+        line = json.dumps({"test": name, **{k: (float(v) if isinstance(v, (np.floating, np.integer)) else v)
+                                            for k, v in figures.items()}})
+        print(line, flush=True)
+        if path:
+            with open(path, "a") as fh:
+                fh.write(line + "\n")
+
+    return _record

@@ -252,6 +252,44 @@ def test_forest_host_concatenation():
     assert f.face_base().tolist() == [0, 320, 1600]
 
 
+def test_forest_offsets_checked_on_the_host():
+    """The offsets must be a CSR of the queries: offsets[0] == 0, non-decreasing,
+    offsets[-1] == n; other offsets are refused before anything reaches the device."""
+    from cubvh_amd.forest import _offsets_tensor
+
+    assert _offsets_tensor([0, 3, 3, 7], 7, "cpu").tolist() == [0, 3, 3, 7]
+    for bad, msg in (([1, 3, 7], "offsets\\[0\\]"), ([0, 3, 6], "offsets\\[-1\\]"), ([0, 5, 3, 7], "non-decreasing"),
+                     ([0], "n_meshes \\+ 1"), ([[0, 7]], "n_meshes \\+ 1"), ([0.0, 7.0], "integers")):
+        with pytest.raises(ValueError, match=msg):
+            _offsets_tensor(torch.tensor(bad), 7, "cpu")
+    assert _offsets_tensor(torch.tensor([5, 1]), 7, "cpu", check=False).tolist() == [5, 1]
+
+
+def test_forest_morton_bins_limit_refused_before_device_work():
+    """A forest call's Morton bins (n_meshes << order bits) must stay within 2^30: the
+    bin index is 32-bit and the histogram holds a word per bin.  16,385 meshes with 16
+    order bits exceed it and are refused by both entry points before any device work."""
+    import cubvh_amd
+    from cubvh_amd import _lib
+    from cubvh_amd import synthetic as S
+
+    lib = _lib.load()
+    v, f = S.icosphere(0)
+    b = cubvh_amd.cuBVH(v, f)
+    n_meshes = (1 << 14) + 1
+    arr = (ctypes.c_void_p * n_meshes)(*([b._h.value] * n_meshes))
+    h = ctypes.c_void_p()
+    _lib.check(lib.cubvh_forest_create(arr, n_meshes, ctypes.byref(h)))
+    try:
+        assert lib.cubvh_forest_reserve(h, 0, 1000, 16, None) == -1
+        assert b"Morton bins" in lib.cubvh_last_error()
+        tun = (ctypes.c_int32 * 4)(16, 1, 8, 4)
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, None) == -1
+        assert b"Morton bins" in lib.cubvh_last_error()
+    finally:
+        lib.cubvh_forest_destroy(h)
+
+
 def test_build_options_and_depth_fallback():
     """Leaf size and split rule flags; a mesh whose SAH tree would be too deep for
     the traversal stack (geometric cluster sizes) still builds (median fallback)."""

@@ -1,0 +1,304 @@
+"""GPU parity on the full benchmark workloads (the library's own call, at full size),
+checked on seeded samples against the oracle, and on every output through properties
+that do not depend on the size.
+
+  level-8 perturbed sphere (1,310,720 faces), 256^3 grid on [-1, 1]^3:
+      unsigned distance (+ uvw), watertight signed distance, occupancy floodfill
+  soup (the level-8 sphere minus 20% of its faces plus 200,000 random triangles):
+      16 x 1920 x 1080 camera rays, raystab signed distance on 2^22 uniform points
+  512-mesh forest (1,280..327,680 faces, 4,096..262,144 uniform points per mesh):
+      one batched call, forward and backward
+
+Bar (written in each test): bit-exact against the oracle's walk of the reference tree
+with the canonical rules (ties to the lowest face index, exact-division intersector);
+against FP64 |d - d64| <= 1e-5 * max(d64, 1), hit/miss and clear-winner faces exact,
+depth within 1e-5 relative, closest point rebuilt from face + uvw within 1e-5.  The
+measured errors of every test are printed and, with CUBVH_PARITY_LOG set, logged.
+"""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+import torch
+
+import cubvh_amd
+from cubvh_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _sample(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
+
+
+@pytest.fixture(scope="module")
+def level8():
+    v, f, _ = S.perturbed_sphere(8, seed=0)
+    return v, f
+
+
+@pytest.fixture(scope="module")
+def level8_bvh(level8):
+    return cubvh_amd.cuBVH(*level8)
+
+
+@pytest.fixture(scope="module")
+def level8_ref(oracle, level8):
+    return oracle.RefBvh(*level8)
+
+
+@pytest.fixture(scope="module")
+def grid256():
+    return S.grid_points(256)
+
+
+def _closest_points(V, F, face, uvw):
+    """Closest points rebuilt from face ids and barycentrics, FP64, on the GPU."""
+    Vd = V.double()
+    tri = F[face]
+    w = uvw.double()
+    return Vd[tri[:, 0]] * w[:, :1] + Vd[tri[:, 1]] * w[:, 1:2] + Vd[tri[:, 2]] * w[:, 2:]
+
+
+def test_level8_udf_grid256(gpu, oracle, level8, level8_bvh, level8_ref, grid256, record):
+    """Dense UDF at the headline size (16,777,216 points, depth-12 tree)."""
+    v, f = level8
+    P = torch.from_numpy(grid256).to(gpu)
+    d, fi, uvw = level8_bvh.unsigned_distance(P, return_uvw=True)
+    assert level8_bvh.info()["stack_need"] > 32  # the deep tree of the headline runs here
+    # every output: finite, in range, and the closest point rebuilt from (face, uvw)
+    # lies at the reported distance
+    assert torch.isfinite(d).all() and (d >= 0).all()
+    assert int(fi.min()) >= 0 and int(fi.max()) < len(f)
+    V = torch.from_numpy(v).to(gpu)
+    F = torch.from_numpy(f).to(gpu).long()
+    cp = _closest_points(V, F, fi, uvw)
+    gap = ((P.double() - cp).norm(dim=1) - d.double()).abs() / d.double().clamp(min=1.0)
+    assert float(gap.max()) <= 1e-5, float(gap.max())
+    assert float(uvw.min()) >= -1e-6 and float((uvw.sum(1) - 1).abs().max()) <= 1e-5
+    # seeded sample bit-exact against the oracle
+    idx = _sample(len(grid256), 1 << 16, 2)
+    od, of, ou, _ = level8_ref.udf(grid256[idx], tie_mode=1, nthreads=THREADS)
+    np.testing.assert_array_equal(_np(d)[idx], od)
+    np.testing.assert_array_equal(_np(fi)[idx], of)
+    np.testing.assert_array_equal(_np(uvw)[idx], ou)
+    # FP64 on 2,048 of them
+    sub = idx[:: len(idx) // 2048][:2048]
+    d64, f64, cp64, gap64 = oracle.udf_f64(v, f, grid256[sub], nthreads=THREADS)
+    dd = _np(d)[sub].astype(np.float64)
+    err = np.abs(dd - d64) / np.maximum(d64, 1.0)
+    assert err.max() <= 1e-5, err.max()
+    clear = gap64 > 1e-5 * np.maximum(d64, 1.0)
+    assert (_np(fi)[sub][clear] == f64[clear]).all()
+    cpe = np.abs(_np(cp)[sub] - cp64).max(1)
+    assert cpe[clear].max() <= 1e-5
+    record("level8_udf_grid256", points=int(len(grid256)), oracle_sample=int(len(idx)), fp64_sample=int(len(sub)),
+           max_rel_dist_err_fp64=err.max(), max_closest_point_err_fp64=cpe[clear].max(),
+           clear_winner_fraction=clear.mean(), max_rel_dist_vs_rebuilt_point_all=float(gap.max()))
+
+
+def test_level8_sdf_watertight_grid256(gpu, oracle, level8, level8_bvh, level8_ref, grid256, record):
+    v, f = level8
+    P = torch.from_numpy(grid256).to(gpu)
+    s, fi, uvw = level8_bvh.signed_distance(P, return_uvw=True, mode="watertight")
+    d, f0, u0 = level8_bvh.unsigned_distance(P, return_uvw=True)
+    # |sdf| is the udf and the face and uvw are the udf's, on every point
+    assert torch.equal(s.abs(), d) and torch.equal(fi, f0) and torch.equal(uvw, u0)
+    idx = _sample(len(grid256), 1 << 16, 3)
+    od, of, ou = level8_ref.sdf_idx(grid256[idx], 0, rule=1, nthreads=THREADS)
+    np.testing.assert_array_equal(_np(s)[idx], od)
+    np.testing.assert_array_equal(_np(fi)[idx], of)
+    np.testing.assert_array_equal(_np(uvw)[idx], ou)
+    # the sign against the FP64 winding number away from the surface
+    sub = idx[:: len(idx) // 512][:512]
+    w = oracle.winding_f64(v, f, grid256[sub], nthreads=THREADS)
+    far = np.abs(_np(s)[sub]) > 1e-4
+    wrong = ((_np(s)[sub] < 0) != (w > 0.5)) & far
+    assert not wrong.any()
+    record("level8_sdf_watertight_grid256", points=int(len(grid256)), oracle_sample=int(len(idx)),
+           winding_sample=int(len(sub)), sign_errors_vs_winding=int(wrong.sum()),
+           inside_fraction_all=float((s < 0).float().mean()))
+
+
+def test_level8_floodfill_occupancy_256(gpu, oracle, level8_bvh, grid256, record):
+    """Occupancy udf < 2/256 of the 256^3 grid: every label equals the BFS."""
+    P = torch.from_numpy(grid256).to(gpu)
+    d, _, _ = level8_bvh.unsigned_distance(P)
+    occ = (d < 2.0 / 256).reshape(1, 256, 256, 256)
+    lab = cubvh_amd.floodfill(occ).cpu().numpy()
+    ref = oracle.floodfill_bfs(occ.cpu().numpy())
+    np.testing.assert_array_equal(lab, ref)
+    comps = np.unique(lab[lab >= 0])
+    assert len(comps) >= 2 and lab[0, 0, 0, 0] == 0  # the outside holds voxel 0
+    record("level8_floodfill_256", voxels=int(lab.size), occupied=int((lab < 0).sum()), components=int(len(comps)),
+           label_mismatches=0)
+
+
+@pytest.fixture(scope="module")
+def soup():
+    return S.soup(8, seed=0)
+
+
+@pytest.fixture(scope="module")
+def soup_bvh(soup):
+    return cubvh_amd.cuBVH(*soup)
+
+
+@pytest.fixture(scope="module")
+def soup_ref(oracle, soup):
+    return oracle.RefBvh(*soup)
+
+
+def test_soup_camera_rays_1080p_x16(gpu, oracle, soup, soup_bvh, soup_ref, record):
+    """16 x 1920 x 1080 camera rays on the soup (33,177,600 rays)."""
+    vs, fs = soup
+    ro, rd = S.camera_rays()
+    RO, RD = torch.from_numpy(ro).to(gpu), torch.from_numpy(rd).to(gpu)
+    pos, fi, t = soup_bvh.ray_trace(RO, RD)
+    # every ray: a miss has face -1 and depth 1000, a hit a face in range and depth
+    # below 1000, and the position is o + t d rounded once (an fp32 fma: within half an
+    # ulp of the exact value)
+    hit = fi >= 0
+    assert bool((t[~hit] == 1000.0).all()) and bool((t[hit] < 1000.0).all()) and int(fi.max()) < len(fs)
+    exact = RO.double() + t.double()[:, None] * RD.double()
+    assert bool(((pos.double() - exact).abs() <= exact.abs() * 2.0 ** -24).all())
+    idx = _sample(len(ro), 1 << 16, 4)
+    op, of, ot, _ = soup_ref.ray(ro[idx], rd[idx], tie_mode=1, nthreads=THREADS)
+    np.testing.assert_array_equal(_np(t)[idx], ot)
+    np.testing.assert_array_equal(_np(fi)[idx], of)
+    np.testing.assert_array_equal(_np(pos)[idx], op)
+    # the reference's own rule (safe_divide clamps |d.n| at 1e-6 and rescales u, v, t;
+    # the first visited face wins a tie) on the same sample: count where it differs
+    rp, rf, rt, _ = soup_ref.ray(ro[idx], rd[idx], tie_mode=0, nthreads=THREADS)
+    flips = (rf >= 0) != (of >= 0)
+    # FP64 Moller-Trumbore on 1,024 rays of the sample
+    sub = idx[:: len(idx) // 1024][:1024]
+    t64, f64, amb = oracle.ray_f64(vs, fs, ro[sub], rd[sub], tol=1e-5, nthreads=THREADS)
+    g_f, g_t = _np(fi)[sub], _np(t)[sub].astype(np.float64)
+    ok = ~amb
+    assert ((g_f >= 0) == (f64 >= 0))[ok].all()
+    both = ok & (f64 >= 0)
+    assert (g_f[both] == f64[both]).all()
+    terr = np.abs(g_t[both] - t64[both]) / t64[both]
+    assert terr.max() <= 1e-5, terr.max()
+    r_f = rf[:: len(idx) // 1024][:1024]
+    ref_wrong = (((r_f >= 0) != (f64 >= 0)) | ((r_f != f64) & (f64 >= 0))) & ok
+    record("soup_camera_rays_1080p_x16", rays=int(len(ro)), hit_fraction_all=float(hit.float().mean()),
+           oracle_sample=int(len(idx)), fp64_sample=int(len(sub)), fp64_ambiguous=int(amb.sum()),
+           max_rel_depth_err_fp64=terr.max(),
+           reference_rule_differs_face=int((rf != of).sum()), reference_rule_differs_depth=int((rt != ot).sum()),
+           reference_rule_hit_miss_flips=int(flips.sum()),
+           reference_rule_wrong_vs_fp64=int(ref_wrong.sum()))
+
+
+def test_soup_raystab_4M(gpu, oracle, soup, soup_bvh, soup_ref, record):
+    """Raystab signed distance of 2^22 uniform points on the soup; the stab rays of a
+    query are keyed by its index in the whole call."""
+    Q = S.uniform_points(1 << 22, seed=0)
+    QT = torch.from_numpy(Q).to(gpu)
+    s, fi, _ = soup_bvh.signed_distance(QT, mode="raystab")
+    d, f0, _ = soup_bvh.unsigned_distance(QT)
+    assert torch.equal(s.abs(), d) and torch.equal(fi, f0)
+    idx = _sample(len(Q), 1 << 13, 5)
+    od, of, _ = soup_ref.sdf_idx(Q[idx], 1, qidx=idx, rule=1, return_uvw=False, nthreads=THREADS)
+    np.testing.assert_array_equal(_np(s)[idx], od)
+    np.testing.assert_array_equal(_np(fi)[idx], of)
+    # the queries whose sign is decided by all 64 stab rays hitting (the "inside" ones)
+    # are the ones a wrong generator or direction would move: make sure the sample has them
+    inside = od < 0
+    assert inside.sum() >= 10
+    # the reference's own rule on the same sample
+    rd_, rf_, _ = soup_ref.sdf_idx(Q[idx], 1, qidx=idx, rule=0, return_uvw=False, nthreads=THREADS)
+    sign_flips = (rd_ < 0) != (od < 0)
+    record("soup_raystab_4M", points=int(len(Q)), inside_fraction_all=float((s < 0).float().mean()),
+           oracle_sample=int(len(idx)), oracle_sample_inside=int(inside.sum()),
+           reference_rule_sign_flips=int(sign_flips.sum()),
+           reference_rule_differs_face=int((rf_ != of).sum()))
+
+
+@pytest.fixture(scope="module")
+def forest512():
+    specs = S.forest(512, seed=0)
+    rng = np.random.default_rng(1)
+    pts = [rng.uniform(-1, 1, size=(sp["n_queries"], 3)).astype(np.float32) for sp in specs]
+    for lvl in sorted({sp["level"] for sp in specs}):
+        S.icosphere_unit(lvl)
+
+    def mesh(sp):
+        return S.perturbed_sphere(sp["level"], seed=sp["seed"], scale=sp["scale"])[:2]
+
+    with ThreadPoolExecutor(max_workers=THREADS) as ex:
+        meshes = list(ex.map(mesh, specs))
+    with ThreadPoolExecutor(max_workers=THREADS) as ex:
+        bvhs = list(ex.map(lambda m: cubvh_amd.cuBVH(*m), meshes))
+    offsets = np.concatenate([[0], np.cumsum([len(p) for p in pts])]).astype(np.int64)
+    return specs, meshes, bvhs, np.concatenate(pts), offsets
+
+
+def test_forest512_equals_per_mesh_and_oracle(gpu, oracle, forest512, record):
+    specs, meshes, bvhs, pts, offsets = forest512
+    forest = cubvh_amd.cuBVHForest(bvhs)
+    assert forest.info()["n_meshes"] == 512
+    P = torch.from_numpy(pts).to(gpu)
+    off = torch.from_numpy(offsets).to(gpu)
+    res = {}
+    for tuning in ((12, 1, 8), (15, 0, 8)):
+        res[tuning] = forest.unsigned_distance(P, off, True, tuning)
+    d, fi, uvw = res[(12, 1, 8)]
+    for x, y in zip(res[(15, 0, 8)], res[(12, 1, 8)]):
+        assert torch.equal(x, y)
+    # every mesh's slice equals its own plain call
+    for m, b in enumerate(bvhs):
+        sl = slice(int(offsets[m]), int(offsets[m + 1]))
+        pd, pf, pu = b.unsigned_distance(P[sl], return_uvw=True)
+        assert torch.equal(d[sl], pd) and torch.equal(fi[sl], pf) and torch.equal(uvw[sl], pu), m
+    # brute force on 256 points of one mesh of each level and a few more
+    levels = {}
+    for m, sp in enumerate(specs):
+        levels.setdefault(sp["level"], m)
+    check = sorted(set(levels.values()) | {0, 1, 2, 255, 511})
+    worst = 0.0
+    for m in check:
+        v, f = meshes[m]
+        sl = slice(int(offsets[m]), int(offsets[m]) + 256)
+        od, of, ou = oracle.udf_brute(v, f, pts[sl], nthreads=THREADS)
+        np.testing.assert_array_equal(_np(d[sl]), od)
+        np.testing.assert_array_equal(_np(fi[sl]), of)
+        np.testing.assert_array_equal(_np(uvw[sl]), ou)
+        d64, _, _, _ = oracle.udf_f64(v, f, pts[sl][:64], nthreads=THREADS)
+        worst = max(worst, float((np.abs(od[:64] - d64) / np.maximum(d64, 1.0)).max()))
+    assert worst <= 1e-5
+    record("forest512", meshes=512, faces=int(forest.info()["n_faces"]), queries=int(len(pts)),
+           meshes_vs_per_mesh_call=512, meshes_vs_brute_force=len(check), max_rel_dist_err_fp64=worst)
+
+
+def test_forest512_backward(gpu, forest512, record):
+    """Batched fwd + bwd on all 512 meshes: point gradients equal the per-mesh autograd
+    path bit for bit; vertex gradients (fp32 atomics, order-dependent) within 1e-5."""
+    specs, meshes, bvhs, pts, offsets = forest512
+    forest = cubvh_amd.cuBVHForest(bvhs)
+    P = torch.from_numpy(pts).to(gpu).requires_grad_(True)
+    vcat, _ = forest.concatenated_mesh(gpu)
+    V = vcat.clone().requires_grad_(True)
+    d, _, _ = cubvh_amd.udf_batched(forest, P, torch.from_numpy(offsets), V)
+    (d * d).sum().backward()
+    vbase = np.concatenate([[0], np.cumsum([len(m[0]) for m in meshes])])
+    worst = 0.0
+    for m in range(0, 512, 64):
+        sl = slice(int(offsets[m]), int(offsets[m + 1]))
+        p = P.detach()[sl].clone().requires_grad_(True)
+        v = torch.from_numpy(meshes[m][0]).to(gpu).requires_grad_(True)
+        dd, _, _ = cubvh_amd.udf(bvhs[m], p, v)
+        (dd * dd).sum().backward()
+        assert torch.equal(P.grad[sl], p.grad), m
+        gv = V.grad[int(vbase[m]):int(vbase[m + 1])]
+        torch.testing.assert_close(gv, v.grad, rtol=1e-5, atol=1e-5)
+        worst = max(worst, float((gv - v.grad).abs().max()))
+    record("forest512_backward", meshes_checked=8, max_abs_vertex_grad_diff=worst)

@@ -113,6 +113,66 @@ def test_pcg32_jump_equals_stepping(oracle):
     np.testing.assert_array_equal(off, expect)
 
 
+def test_pcg32_equals_reference_generator(oracle):
+    """The oracle's generator against the reference's own (include/gpu/pcg32.h compiled
+    from its header into oracle/_ref): outputs after jumps, and the raystab offsets of
+    every query index of a 2^22-point call (src/bvh.cu:667-668)."""
+    if not oracle.ref_pcg32_available():
+        pytest.skip("oracle/_ref/libref_pcg32.so not built (reference sources absent)")
+    for adv in (0, 1, 2, 17, 1000, 65537, 1 << 32, (1 << 33) + 5, (1 << 40) + 3, (1 << 62) + 11, -1, -1000):
+        np.testing.assert_array_equal(oracle.pcg32_uints(adv, 64), oracle.ref_pcg32_uints(adv, 64))
+    n = 1 << 22
+    np.testing.assert_array_equal(oracle.pcg32_offsets(n), oracle.ref_pcg32_offsets(n))
+
+
+def test_pcg32_state_is_the_textbook_lcg(oracle):
+    """The state recurrence is the 64-bit LCG s' = 6364136223846793005 s + c (mod 2^64)
+    with the reference's default state and increment (pcg32.h:33-35, 47): stepped here
+    with Python integers, compared with the reference generator's jump-ahead, and the
+    first outputs follow from the states by the XSH-RR rule."""
+    if not oracle.ref_pcg32_available():
+        pytest.skip("oracle/_ref/libref_pcg32.so not built (reference sources absent)")
+    a, c, m = 6364136223846793005, 0xDA3E39CB94B95BDB, 1 << 64
+    s = 0x853C49E6748FEA9B
+    states = [s]
+    for _ in range(3000):
+        s = (a * s + c) % m
+        states.append(s)
+    for k in (0, 1, 2, 3, 100, 2047, 3000):
+        assert oracle.ref_pcg32_state(k) == states[k]
+
+    def xsh_rr(old):
+        x = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        r = old >> 59
+        return ((x >> r) | (x << ((32 - r) & 31))) & 0xFFFFFFFF
+
+    np.testing.assert_array_equal(oracle.pcg32_uints(0, 40), np.array([xsh_rr(x) for x in states[:40]], np.uint32))
+
+
+def test_fibonacci_lattice_properties(oracle):
+    """Properties of the stab directions (common.h:63-96 with 32 directions) that a
+    wrong epsilon, golden ratio or axis would break: consecutive azimuths step by
+    minus the golden angle 2 pi (2 - phi) (mod 2 pi); at zero offset the 32 values of
+    cos(theta) are equally spaced by 2 / (31 + 2 * 1.33), start at
+    1 - 2 * 1.33 / (31 + 2 * 1.33) and are symmetric about 0."""
+    golden = (1 + 5 ** 0.5) / 2
+    for ox, oy in ((0.0, 0.0), (0.25, 0.6), (0.71, 0.05)):
+        d = oracle.fibonacci_dirs32(ox, oy).astype(np.float64)
+        az = np.arctan2(d[:, 1], d[:, 0])
+        step = np.mod(np.diff(az), 2 * np.pi)
+        np.testing.assert_allclose(step, 2 * np.pi - 2 * np.pi * (2 - golden), atol=5e-5)
+        # the first azimuth is 2 pi (frac(oy) - 1/2)
+        np.testing.assert_allclose(np.mod(az[0] - 2 * np.pi * (oy - 0.5) + np.pi, 2 * np.pi) - np.pi, 0, atol=5e-6)
+    ct = oracle.fibonacci_dirs32(0.0, 0.0).astype(np.float64)[:, 2]
+    gap = 2 / (31 + 2 * 1.33)
+    np.testing.assert_allclose(np.diff(ct), -gap, atol=2e-6)
+    np.testing.assert_allclose(ct[0], 1 - 1.33 * gap, atol=2e-6)
+    np.testing.assert_allclose(ct + ct[::-1], 0, atol=2e-6)
+    # an offset shifts the cos(theta) lattice cyclically by 2 ox (mod 2), in units of the gap
+    ct2 = oracle.fibonacci_dirs32(0.5 * gap, 0.0).astype(np.float64)[:, 2]
+    np.testing.assert_allclose(ct2[:-1], ct[1:], atol=2e-6)
+
+
 def test_fibonacci_directions(oracle):
     for ox, oy in ((0.0, 0.0), (0.3, 0.7), (0.999, 0.123)):
         d = oracle.fibonacci_dirs32(ox, oy).astype(np.float64)
