@@ -1,7 +1,9 @@
 """Build recipe for libcubvh_amd.so (the C-ABI BVH query library, gfx950).
 
 Compiles every source under cubvh_amd/csrc with hipcc into cubvh_amd/libcubvh_amd.so,
-in-tree.  Run ``python -m cubvh_amd.build`` or call :func:`build_library`.
+in-tree, and the `_cubvh` extension module (csrc/bindings.cpp) against torch and that
+library.  Run ``python -m cubvh_amd.build`` or call :func:`build_library` and
+:func:`build_binding`.
 """
 from __future__ import annotations
 
@@ -92,6 +94,49 @@ def build_library(verbose: bool = False, jobs: int = 4) -> Path:
     return LIB_PATH
 
 
+BINDING_SRC = "bindings.cpp"
+
+
+def binding_path() -> Path:
+    """In-tree path of the `_cubvh` extension module (importable as cubvh_amd._cubvh,
+    and as top-level `_cubvh` with cubvh_amd/ on sys.path)."""
+    import sysconfig
+
+    return PKG_DIR / ("_cubvh" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_binding(verbose: bool = False) -> Path:
+    """Compile the `_cubvh` PyTorch-ROCm extension (csrc/bindings.cpp: the reference
+    package's native module names over the C ABI) against torch's headers and libraries
+    and libcubvh_amd.so (found next to it through $ORIGIN).  Always rebuilds."""
+    import torch
+    from torch.utils import cpp_extension
+
+    cc = hipcc()
+    out = binding_path()
+    tmp = out.with_name(out.name + ".tmp")
+    torch_lib = cpp_extension.library_paths()[0]
+    inc = [f"-I{p}" for p in cpp_extension.include_paths()]
+    import sysconfig
+
+    inc.append(f"-I{sysconfig.get_paths()['include']}")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [cc, "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", "-DUSE_ROCM=1",
+           "-DTORCH_EXTENSION_NAME=_cubvh", "-DTORCH_API_INCLUDE_EXTENSION_H", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-Wno-unused-result", "-Wno-deprecated-declarations", *inc, "-I", str(REPO_DIR / "include"),
+           str(CSRC / BINDING_SRC), "-o", str(tmp),
+           f"-L{torch_lib}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip", "-ltorch_python",
+           f"-L{PKG_DIR}", "-lcubvh_amd", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{torch_lib}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("_cubvh extension build failed:\n" + r.stdout)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     path = build_library(verbose="-v" in sys.argv)
     print(path)
+    print(build_binding(verbose="-v" in sys.argv))
