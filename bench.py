@@ -49,16 +49,55 @@ def _algo_bytes():
     return json.loads(p.read_text())
 
 
-def _traffic_per_launch():
-    """HBM traffic of the headline kernel from the committed counter summary."""
-    p = ROOT / "profiles" / "udf_level8_grid256_traffic.md"
+def profile_figures(name):
+    """Figures of a committed counter summary (profiles/<name>_counters.md): every line
+    of the form `key: number`.  Empty when the file is absent."""
+    out = {}
+    p = ROOT / "profiles" / f"{name}_counters.md"
     try:
-        for line in p.read_text().splitlines():
-            if "traffic_bytes_per_launch:" in line:
-                return float(line.split(":")[1])
-    except (OSError, ValueError):
-        pass
-    return None
+        text = p.read_text()
+    except OSError:
+        return out
+    for line in text.splitlines():
+        key, sep, val = line.partition(":")
+        key = key.strip().lstrip("*- ").strip()
+        if not sep or not key.replace("_", "").isalnum() or " " in key:
+            continue
+        try:
+            out[key] = float(val.strip().split()[0])
+        except (ValueError, IndexError):
+            pass
+    return out
+
+
+def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None, copy_gbs=None):
+    """Roofline of one kernel: HBM bytes per launch from the counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy of known size) over `kernel_s`, the launch's average
+    duration measured here; the VALU figures of the same summary (busy = share of the
+    kernel's cycles the vector ALUs issue, lanes = active lanes per VALU instruction);
+    the algorithmic bytes (per-unit figure x units) over the same time when given."""
+    f = profile_figures(name)
+    traffic = f.get("traffic_bytes_per_launch")
+    achieved = traffic / kernel_s / 1e9 if traffic and kernel_s > 0 else None
+    blk = {"kernel": kernel, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+           "achieved": achieved, "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
+           "traffic": traffic, "kernel_ms": kernel_s * 1e3,
+           "valu_busy": f.get("valu_busy"), "valu_lane_utilization": f.get("valu_lane_utilization"),
+           "l2_hit_rate": f.get("l2_hit_rate"), "counters": f"profiles/{name}_counters.md" if f else None}
+    if copy_gbs and achieved is not None:
+        blk["copy_GBs"] = copy_gbs
+        blk["frac_of_copy"] = achieved / copy_gbs
+    if units is not None and algo_bytes_per_unit is not None:
+        algo = algo_bytes_per_unit * units / kernel_s / 1e9
+        blk["algorithmic_bytes_per_unit"] = algo_bytes_per_unit
+        blk["algorithmic_GBs"] = algo
+        blk["algorithmic_frac"] = algo / HBM_PEAK_GBS
+    busy = blk["valu_busy"]
+    if busy is not None and blk["frac"] is not None and busy > blk["frac"]:
+        blk["limiter"] = "valu issue (valu_busy above the HBM fraction)"
+    elif blk["frac"] is not None:
+        blk["limiter"] = "hbm"
+    return blk
 
 
 def _time_kernel(fn, steps, warmup):
@@ -157,9 +196,29 @@ def fwd_bwd_leg(dev, steps, warmup):
 
     _, tf = _time_kernel(fwd, steps, warmup)
     _, tfb = _time_kernel(fwdbwd, steps, warmup)
+    # the backward kernel alone, through the C ABI on the current stream: dL/dd of
+    # mean(d^2) = 2 d / n, gradients of the points and (accumulated) of the vertices
+    import ctypes
+
+    lib = cubvh_amd._lib.load()
+    d, fi, uvw = bvh6.unsigned_distance(P, return_uvw=True)
+    g = d * (2.0 / n)
+    F6 = torch.from_numpy(f6).to(dev)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(V6)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+    def bwd():
+        cubvh_amd._lib.check(lib.cubvh_udf_backward(dev.index, n, g.data_ptr(), P.data_ptr(), fi.data_ptr(),
+                                                    uvw.data_ptr(), V6.data_ptr(), V6.shape[0], F6.data_ptr(),
+                                                    F6.shape[0], gp.data_ptr(), gv.data_ptr(), stream))
+
+    _, tb = _time_kernel(bwd, steps, warmup)
     return {"workload": "jittered icosphere level 6 (81,920 faces), 2^20 points N(0, 0.4^2), loss mean(udf^2)",
             "queries": n, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf,
-            "fwd_Mq_s": n / tf / 1e6, "fwd_bwd_Mq_s": n / tfb / 1e6}
+            "fwd_Mq_s": n / tf / 1e6, "fwd_bwd_Mq_s": n / tfb / 1e6, "bwd_kernel_ms": tb * 1e3,
+            "roofline_bwd": roofline_block("udf_backward_level6", "udf_backward_kernel", tb, units=n,
+                                           algo_bytes_per_unit=168)}
 
 
 def batched_leg(dev, steps, warmup, n_meshes=512):
@@ -212,6 +271,17 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
                                steps, warmup)
     graph = cubvh_amd.ForestGraph(forest, P, off, tuning=tuning)
     _, t_graph = _time_kernel(graph.replay, steps, warmup)
+    # batched forward + hand-written backward (loss sum(d^2) over every mesh's queries)
+    vcat, _ = forest.concatenated_mesh(dev)
+
+    def fwdbwd():
+        p = P.detach().requires_grad_(True)
+        vv = vcat.detach().requires_grad_(True)
+        d, _, _ = cubvh_amd.udf_batched(forest, p, off_host, vv, tuning=tuning)
+        (d * d).sum().backward()
+
+    off_host = torch.from_numpy(offsets)
+    _, t_fwdbwd = _time_kernel(fwdbwd, steps, warmup)
     # the batched answers equal the per-mesh answers
     ok = True
     for m in range(0, n_meshes, 37):
@@ -247,7 +317,10 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
             "host_build_s": build_s, "gpu_build_s": gpu_build_s, "gpu_built_forest_Mq_s": n / t_gforest / 1e6,
             "plain_ms": t_plain * 1e3, "forest_ms": t_forest * 1e3, "graph_ms": t_graph * 1e3,
             "plain_Mq_s": n / t_plain / 1e6, "batched_Mq_s": n / best / 1e6, "speedup": t_plain / best,
-            "tuning": list(tuning), "equal_to_per_mesh": ok}
+            "fwd_bwd_ms": t_fwdbwd * 1e3, "fwd_bwd_Mq_s": n / t_fwdbwd / 1e6, "fwd_bwd_ratio": t_fwdbwd / t_forest,
+            "tuning": list(tuning), "equal_to_per_mesh": ok,
+            "roofline": dict(roofline_block("forest512", "forest_distance_kernel", t_forest),
+                             time_basis="whole forest call (per-mesh Morton ordering + walk)")}
 
 
 def hashtable_leg(dev, steps, warmup, with_cpu=True):
@@ -383,7 +456,10 @@ def run_extras(dev, steps, warmup, out):
     bvh8 = cubvh_amd.cuBVH(v8, f8)
     G = torch.from_numpy(S.grid_points(256)).to(dev)
     _, t = _time_kernel(lambda: bvh8.signed_distance(G, mode="watertight"), steps, 1)
-    out["sdf_watertight_grid256"] = {"Mq_s": G.shape[0] / t / 1e6, "ms": t * 1e3}
+    out["sdf_watertight_grid256"] = {"Mq_s": G.shape[0] / t / 1e6, "ms": t * 1e3,
+                                     "roofline": dict(roofline_block("sdf_watertight_level8_grid256",
+                                                                     "distance_packet_kernel<1>", t),
+                                                      time_basis="whole call (Morton ordering + walk)")}
 
     # the same mesh built on the GPU (linear BVH): build time and UDF rate on that tree
     V8 = torch.from_numpy(v8).to(dev)
@@ -414,7 +490,10 @@ def run_extras(dev, steps, warmup, out):
     occ = (d < 2.0 / 256).reshape(1, 256, 256, 256)
     _, t = _time_kernel(lambda: cubvh_amd.floodfill(occ), steps, 1)
     out["floodfill_256"] = {"ms": t * 1e3, "Mvoxels_s": occ.numel() / t / 1e6,
-                            "GB_s_5B_per_voxel": 5 * occ.numel() / t / 1e9}
+                            "GB_s_5B_per_voxel": 5 * occ.numel() / t / 1e9,
+                            "roofline": dict(roofline_block("floodfill_grid256", "ff_tiles + 3 ff_merge_axis + ff_flatten",
+                                                            t, units=occ.numel(), algo_bytes_per_unit=5),
+                                             time_basis="whole call (5 kernel launches)")}
     del G, d, occ
 
     # soup: camera rays (16 x 1920 x 1080) and raystab on 2^22 points
@@ -426,11 +505,17 @@ def run_extras(dev, steps, warmup, out):
     RO = torch.from_numpy(ro).to(dev)
     RD = torch.from_numpy(rd).to(dev)
     _, t = _time_kernel(lambda: bvhs.ray_trace(RO, RD), max(2, steps // 2), 1)
-    out["rays_soup_1080p_x16"] = {"Mrays_s": RO.shape[0] / t / 1e6, "ms": t * 1e3}
+    out["rays_soup_1080p_x16"] = {"Mrays_s": RO.shape[0] / t / 1e6, "ms": t * 1e3,
+                                  "roofline": dict(roofline_block("rays_soup_1080p_x16", "ray_packet_kernel", t),
+                                                   time_basis="whole call (one kernel launch)")}
     del RO, RD
     Q = torch.from_numpy(S.uniform_points(1 << 22, seed=0)).to(dev)
     _, t = _time_kernel(lambda: bvhs.signed_distance(Q, mode="raystab"), max(2, steps // 2), 1)
-    out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3}
+    _, tu = _time_kernel(lambda: bvhs.unsigned_distance(Q), max(2, steps // 2), 1)
+    # the stab rays' kernel: the raystab call minus the unsigned call it starts with
+    out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3, "udf_part_ms": tu * 1e3,
+                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_sign_kernel", max(t - tu, 1e-9)),
+                                               time_basis="raystab call minus the unsigned call")}
 
 
 def main():
@@ -525,14 +610,15 @@ def main():
     dropin_wall, _ = _time_kernel(dropin, args.steps, 1)
     ab = _algo_bytes()
     roof = None
-    if ab and "udf_level8_grid256" in ab["workloads"] and args.res == 256:
-        bpq = ab["workloads"]["udf_level8_grid256"]["bytes_per_query"]
-        achieved = bpq * n / ev / 1e9
+    if args.res == 256:
         copy_gbs = copy_bandwidth(dev)
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic_per_launch(),
-                "bytes_per_query": bpq, "kernel_ms": ev * 1e3,
-                "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
+        roof = roofline_block("udf_level8_grid256", "distance_packet_kernel<0>", ev, copy_gbs=copy_gbs)
+        if ab and "udf_level8_grid256" in ab["workloads"]:
+            # the reference's own walk (bvh.cu:300-353) on this workload, priced at 24 B per
+            # box test and 36 B per triangle test: the work this walk replaces, not HBM traffic
+            bpq = ab["workloads"]["udf_level8_grid256"]["bytes_per_query"]
+            roof["reference_walk_bytes_per_query"] = bpq
+            roof["reference_walk_equiv_GBs"] = bpq * n / ev / 1e9
     result = {
         "metric": "M UDF queries/s @1.3M tris 256^3 (plain vs fwd+bwd vs batched), % HBM roofline",
         "value": value,
@@ -562,6 +648,17 @@ def main():
         result["extra"] = extra
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(v, f, pts)
+        # the timed pass's own answers on a seeded sample against the oracle (canonical ties)
+        from oracle import oracle as O
+
+        idx = np.sort(np.random.default_rng(7).choice(n, 1 << 16, replace=False))
+        od, of, _, _ = O.RefBvh(v, f).udf(pts[idx], return_uvw=False, tie_mode=1,
+                                          nthreads=min(os.cpu_count() or 1, 16))
+        gd = dist.cpu().numpy()[idx]
+        gf = face.cpu().numpy()[idx]
+        result["parity"] = {"sample": f"{len(idx)} seeded points of the timed pass's output vs the oracle",
+                            "distance_mismatches": int((gd != od).sum()), "face_mismatches": int((gf != of).sum()),
+                            "max_abs_distance_diff": float(np.abs(gd.astype(np.float64) - od).max())}
     print(json.dumps(result), flush=True)
 
 

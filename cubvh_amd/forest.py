@@ -198,5 +198,6 @@ def udf_batched(forest: cuBVHForest, points, offsets, vertices=None, tuning=None
         v = v.to(device=dev, dtype=torch.float32)
     v = v.contiguous()
     counts = (offsets[1:] - offsets[:-1]).to(torch.int64)
-    fb = torch.repeat_interleave(forest.face_base().to(dev), counts)
+    # output_size: the total is known, so no device synchronization
+    fb = torch.repeat_interleave(forest.face_base().to(dev), counts, output_size=p.shape[0])
     return _BatchedUDF.apply(p, v, forest, offsets, fcat, fb, tuning if tuning is not None else DEFAULT_TUNING)

