@@ -31,6 +31,9 @@ SIGNATURES = {
     "cubvh_bvh_triangles": (c_int, [c_void_p, c_void_p]),
     "cubvh_bvh_upload": (c_int, [c_void_p, c_int, c_void_p]),
     "cubvh_unsigned_distance": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "cubvh_unsigned_distance_noalloc": (
+        c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    ),
     "cubvh_morton_order": (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p]),
     "cubvh_distance_ordered": (
         c_int,

@@ -186,6 +186,13 @@ struct cubvh_bvh {
         return ensure_host_derived_locked();
     }
 
+    // Whether the tree is already on `device` (uploaded or built there).
+    bool resident(int device) {
+        if (device < 0 || device >= kMaxDevices) return false;
+        std::lock_guard<std::mutex> lock(mu);
+        return dev[device].block != nullptr;
+    }
+
     // Uploads on first use (the reference's lazy upload, src/api_gpu.cu:76-78).
 
     int ensure_device(int device, hipStream_t stream) {
@@ -732,6 +739,48 @@ int cubvh_bvh_upload(cubvh_bvh* bvh, int device, void* stream) {
 int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* positions, float* distances,
                             int64_t* face_id, float* uvw, void* stream) {
     return run_distance(bvh, device, n, 0, positions, distances, face_id, uvw, stream);
+}
+
+int cubvh_unsigned_distance_noalloc(cubvh_bvh* bvh, int device, int64_t n, const float* positions,
+                                    float* distances, int64_t* face_id, float* uvw, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!bvh) return fail(CUBVH_ERR_ARG, "bvh is null");
+    if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
+    if (n == 0) return CUBVH_OK;
+    if (!positions || !distances || !face_id) return fail(CUBVH_ERR_ARG, "null query buffer");
+    if (n > kNoAllocMaxQueries)
+        return fail(CUBVH_ERR_ARG, "at most 2^20 queries per allocation-free call (the size of its static scratch)");
+    if (device < 0 || device >= kMaxDevices || device >= kNoAllocDevices)
+        return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    if (!bvh->resident(device))
+        return fail(CUBVH_ERR_STATE, "the tree is not on this device: call cubvh_bvh_upload first "
+                                     "(the allocation-free call does not upload it)");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    QueryArgs q = bvh->args(device, n, stream);
+    q.in0 = positions;
+    q.out_f = distances;
+    q.out_id = face_id;
+    q.out_v = uvw;
+    q.packet = false;
+    hipStream_t hs = (hipStream_t)stream;
+    if (n >= kMortonMin) {
+        void* ws = nullptr;
+        hipError_t e = noalloc_scratch(device, &ws);
+        if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)");
+        const size_t ob = ((size_t)n * 4 + 255) & ~(size_t)255;
+        if (ob + morton_workspace_bytes(n) > kNoAllocScratchBytes)
+            return fail(CUBVH_ERR_STATE, "the ordering workspace does not fit the static scratch");
+        uint32_t* order = (uint32_t*)ws;
+        e = morton_order(n, positions, (char*)ws + ob, order, hs);
+        if (e != hipSuccess) return fail_hip(e, "query ordering");
+        q.order = order;
+        q.packet = true;
+    }
+    hipError_t e = launch_distance(q, 0);
+    if (e != hipSuccess) return fail_hip(e, "distance kernel launch");
+    return CUBVH_OK;
+    CUBVH_GUARD_END
 }
 
 int cubvh_morton_order(int device, int64_t n, const float* positions, uint32_t* order, void* stream) {

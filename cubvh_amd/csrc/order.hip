@@ -14,6 +14,8 @@
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "query.hpp"
 
 namespace cubvh_amd {
@@ -125,6 +127,34 @@ size_t morton_workspace_bytes(int64_t n) {
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     return al(64) + 3 * al(4 * (size_t)n) + al(tmp);
+}
+
+// Scratch of the allocation-free distance entry point (cubvh_unsigned_distance_noalloc):
+// the Morton permutation of up to kNoAllocMaxQueries queries and the ordering
+// workspace, in a static device array.  Every device that runs the library holds its
+// own copy until the process ends.  Its host-side name is not a device address: the
+// launches take the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, which must be the current device.  Looked
+// up once per device and kept.
+hipError_t noalloc_scratch(int device, void** ptr) {
+    if (device < 0 || device >= kNoAllocDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 // Writes the Morton permutation of the n queries into order_out (n uint32) using

@@ -46,6 +46,14 @@ hipError_t launch_stab_sign(const QueryArgs& q);
 size_t morton_workspace_bytes(int64_t n);
 hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* order_out, hipStream_t s);
 
+// Static per-device scratch of the allocation-free distance call (order.hip): room for
+// the Morton order and ordering workspace of kNoAllocMaxQueries queries.
+constexpr int64_t kNoAllocMaxQueries = (int64_t)1 << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kNoAllocDevices = 64;
+// Device address of the scratch on `device` (the current device); looked up once per device.
+hipError_t noalloc_scratch(int device, void** ptr);
+
 hipError_t launch_floodfill(const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D, int32_t* labels,
                             hipStream_t stream);
 

@@ -129,6 +129,19 @@ int cubvh_bvh_upload(cubvh_bvh* bvh, int device, void* stream);
 int cubvh_unsigned_distance(cubvh_bvh* bvh, int device, int64_t n, const float* positions, float* distances,
                             int64_t* face_id, float* uvw, void* stream);
 
+/* No counterpart in the reference: cubvh_unsigned_distance for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results, bit for bit, as
+ * cubvh_unsigned_distance; at most 2^20 queries per call (larger n is refused).  The
+ * Morton order of the queries and its workspace live in a static 32 MiB device array
+ * that each GPU running the library holds until the process ends; the call allocates
+ * and frees nothing.  The tree must already be on `device` (cubvh_bvh_upload or an
+ * earlier query there), otherwise the call fails.  Every call on one GPU uses that
+ * GPU's one array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialized by the caller. */
+int cubvh_unsigned_distance_noalloc(cubvh_bvh* bvh, int device, int64_t n, const float* positions,
+                                    float* distances, int64_t* face_id, float* uvw, void* stream);
+
 /* Replaces cuBVH::signed_distance (src/api_gpu.cu:97-108, src/bvh.cu:430-466,
  * kernels src/bvh.cu:631-685).  mode 0 = watertight, any other value = raystab.
  * The raystab random offsets are keyed by the query index within this call. */
