@@ -217,11 +217,13 @@ def fwd_bwd_leg(dev, steps, warmup):
     return {"workload": "jittered icosphere level 6 (81,920 faces), 2^20 points N(0, 0.4^2), loss mean(udf^2)",
             "queries": n, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf,
             "fwd_Mq_s": n / tf / 1e6, "fwd_bwd_Mq_s": n / tfb / 1e6, "bwd_kernel_ms": tb * 1e3,
-            "roofline_bwd": roofline_block("udf_backward_level6", "udf_backward_kernel", tb, units=n,
-                                           algo_bytes_per_unit=168)}
+            "roofline_bwd": dict(roofline_block("udf_backward_level6",
+                                                "query_terms + face_sums + corner_keys + vertex_sums + 2 radix sorts",
+                                                tb, units=n, algo_bytes_per_unit=168),
+                                 time_basis="whole cubvh_udf_backward call")}
 
 
-def batched_leg(dev, steps, warmup, n_meshes=512):
+def batched_leg(dev, steps, warmup, n_meshes=512, gpu_build=True):
     """512 meshes with uneven query sets: per-mesh calls vs one forest launch."""
     import torch
 
@@ -290,6 +292,18 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
         ok &= bool(torch.equal(graph.face[offsets[m]:offsets[m + 1]], f_ref))
     best = min(t_forest, t_graph)
     del graph
+    res = {"workload": f"{n_meshes} perturbed icospheres, {int(forest.info()['n_faces'])} faces, "
+                       f"{n} uniform points in [-1,1]^3 (4,096..262,144 per mesh)",
+           "queries": int(n), "faces": int(forest.info()["n_faces"]), "mesh_generation_s": gen_s,
+           "host_build_s": build_s,
+           "plain_ms": t_plain * 1e3, "forest_ms": t_forest * 1e3, "graph_ms": t_graph * 1e3,
+           "plain_Mq_s": n / t_plain / 1e6, "batched_Mq_s": n / best / 1e6, "speedup": t_plain / best,
+           "fwd_bwd_ms": t_fwdbwd * 1e3, "fwd_bwd_Mq_s": n / t_fwdbwd / 1e6, "fwd_bwd_ratio": t_fwdbwd / t_forest,
+           "tuning": list(tuning), "equal_to_per_mesh": ok,
+           "roofline": dict(roofline_block("forest512", "forest_distance_kernel", t_forest),
+                            time_basis="whole forest call (per-mesh Morton ordering + walk)")}
+    if not gpu_build:
+        return res
     # the same meshes built on the GPU (already resident there) and assembled on the device
     mesh_dev = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f in meshes]
     torch.cuda.synchronize()
@@ -302,25 +316,19 @@ def batched_leg(dev, steps, warmup, n_meshes=512):
     gpu_build_s = time.perf_counter() - t0
     stream = torch.cuda.current_stream(dev).cuda_stream
     cubvh_amd._lib.check(lib.cubvh_forest_reserve(gforest, dev.index, n, tuning[0], stream))
-    tun = (ctypes.c_int32 * 3)(*tuning)
+    tun = (ctypes.c_int32 * len(tuning))(*tuning)
 
     def gpu_forest():
         cubvh_amd._lib.check(lib.cubvh_forest_unsigned_distance(gforest, dev.index, n, P.data_ptr(), off.data_ptr(),
-                                                                dist.data_ptr(), face.data_ptr(), None, tun, stream))
+                                                                dist.data_ptr(), face.data_ptr(), None, tun,
+                                                                len(tuning), stream))
 
     _, t_gforest = _time_kernel(gpu_forest, steps, warmup)
     lib.cubvh_forest_destroy(gforest)
     del gbvhs
-    return {"workload": f"{n_meshes} perturbed icospheres, {int(forest.info()['n_faces'])} faces, "
-                        f"{n} uniform points in [-1,1]^3 (4,096..262,144 per mesh)",
-            "queries": int(n), "faces": int(forest.info()["n_faces"]), "mesh_generation_s": gen_s,
-            "host_build_s": build_s, "gpu_build_s": gpu_build_s, "gpu_built_forest_Mq_s": n / t_gforest / 1e6,
-            "plain_ms": t_plain * 1e3, "forest_ms": t_forest * 1e3, "graph_ms": t_graph * 1e3,
-            "plain_Mq_s": n / t_plain / 1e6, "batched_Mq_s": n / best / 1e6, "speedup": t_plain / best,
-            "fwd_bwd_ms": t_fwdbwd * 1e3, "fwd_bwd_Mq_s": n / t_fwdbwd / 1e6, "fwd_bwd_ratio": t_fwdbwd / t_forest,
-            "tuning": list(tuning), "equal_to_per_mesh": ok,
-            "roofline": dict(roofline_block("forest512", "forest_distance_kernel", t_forest),
-                             time_basis="whole forest call (per-mesh Morton ordering + walk)")}
+    res["gpu_build_s"] = gpu_build_s
+    res["gpu_built_forest_Mq_s"] = n / t_gforest / 1e6
+    return res
 
 
 def hashtable_leg(dev, steps, warmup, with_cpu=True):
@@ -445,7 +453,7 @@ def smc_leg(dev, steps, warmup, with_cpu=True):
     return res
 
 
-def run_extras(dev, steps, warmup, out):
+def run_extras(dev, steps, warmup, out, gpu_build=True):
     import torch
 
     import cubvh_amd
@@ -468,8 +476,9 @@ def run_extras(dev, steps, warmup, out):
     import ctypes
 
     stream = torch.cuda.current_stream(dev).cuda_stream
-    for name, flags, build in (("gpu_build_level8", 0, "gpu"), ("gpu_build_level8_sah", 0x200, "gpu_sah"),
-                               ("gpu_build_level8_lbvh", 0x100, "gpu_lbvh")):
+    builds = (("gpu_build_level8", 0, "gpu"), ("gpu_build_level8_sah", 0x200, "gpu_sah"),
+              ("gpu_build_level8_lbvh", 0x100, "gpu_lbvh"))
+    for name, flags, build in builds if gpu_build else ():
         times = []
         for _ in range(3):
             h = ctypes.c_void_p()
@@ -514,7 +523,7 @@ def run_extras(dev, steps, warmup, out):
     _, tu = _time_kernel(lambda: bvhs.unsigned_distance(Q), max(2, steps // 2), 1)
     # the stab rays' kernel: the raystab call minus the unsigned call it starts with
     out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3, "udf_part_ms": tu * 1e3,
-                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_sign_kernel", max(t - tu, 1e-9)),
+                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_init + stab_first + stab_rest + stab_negate", max(t - tu, 1e-9)),
                                                time_basis="raystab call minus the unsigned call")}
 
 
@@ -528,6 +537,8 @@ def main():
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--leaf", type=int, default=0, help="leaf_size argument of cuBVH (0 = library default)")
+    ap.add_argument("--no-gpu-build", action="store_true",
+                    help="skip the GPU tree builds of the batched and extra legs (for counter passes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device-to-device tensor copy first (reference for HBM counters)")
     ap.add_argument("--hashtable-only", action="store_true",
@@ -639,12 +650,12 @@ def main():
     }
     if not args.headline_only:
         result["fwd_bwd"] = fwd_bwd_leg(dev, max(3, args.steps), args.warmup)
-        result["batched"] = batched_leg(dev, max(2, args.steps // 3), 1)
+        result["batched"] = batched_leg(dev, max(2, args.steps // 3), 1, gpu_build=not args.no_gpu_build)
         result["hashtable"] = hashtable_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
         result["sparse_marching_cubes"] = smc_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
     extra = {}
     if args.extras:
-        run_extras(dev, max(2, args.steps // 2), 1, extra)
+        run_extras(dev, max(2, args.steps // 2), 1, extra, gpu_build=not args.no_gpu_build)
         result["extra"] = extra
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(v, f, pts)

@@ -54,7 +54,7 @@ SIGNATURES = {
     "cubvh_forest_reserve": (c_int, [c_void_p, c_int, c_int64, c_int32, c_void_p]),
     "cubvh_forest_unsigned_distance": (
         c_int,
-        [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+        [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p],
     ),
     "cubvh_udf_backward": (
         c_int,

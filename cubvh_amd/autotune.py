@@ -1,11 +1,16 @@
 """Autotuner of the batched forest launch.
 
-Tries the launch configurations of :meth:`cuBVHForest.unsigned_distance` (Morton-
-prefix bits used to order the queries inside each mesh, persistent grid or one wave
-per tile, waves per CU of the persistent grid) on the caller's own batch, times each
-with HIP events and caches the winner per (device, query-count bucket, mesh-count
-bucket, face-count bucket) in a JSON file (``$CUBVH_AMD_TUNE_CACHE`` or
-``~/.cache/cubvh_amd/autotune.json``).
+Tries the launch configurations of :meth:`cuBVHForest.unsigned_distance` on the
+caller's own batch -- Morton-prefix bits used to order the queries inside each mesh
+(the query-sort granularity), persistent grid or one wave per tile, waves per CU of
+the persistent grid, then waves per workgroup (the block size: 64 to 512 lanes) for
+the best of those -- times each with HIP events and caches the winner per (device,
+query-count bucket, mesh-count bucket, face-count bucket) in a JSON file
+(``$CUBVH_AMD_TUNE_CACHE`` or ``~/.cache/cubvh_amd/autotune.json``).
+
+The traversal stack of the forest's packet walk is not a knob: it holds one entry per
+pending subtree in three registers spread over the wave's 64 lanes, so its depth is
+the tree's (at most 3 x depth + 1 <= 64 entries), whatever the launch shape.
 """
 from __future__ import annotations
 
@@ -17,10 +22,12 @@ from pathlib import Path
 import torch
 
 CANDIDATES = [
-    (bits, persistent, wpc)
+    (bits, persistent, wpc, 4)
     for bits in (0, 6, 9, 12, 15)
     for persistent, wpc in ((0, 8), (1, 4), (1, 8), (1, 16))
 ]
+WAVES_PER_BLOCK = (1, 2, 4, 8)
+MAX_BINS = 1 << 30  # n_meshes << bits (the library refuses more Morton bins)
 
 
 def _cache_path() -> Path:
@@ -68,13 +75,17 @@ def tune_forest(forest, points, offsets, candidates=None, reps=3, use_cache=True
     cache = _load() if use_cache else {}
     if use_cache and key in cache:
         return tuple(cache[key]["best"]), cache[key]["ms"]
-    cands = candidates or CANDIDATES
+    n_meshes = forest.info()["n_meshes"]
+    cands = [tuple(c) for c in (candidates or CANDIDATES) if (n_meshes << c[0]) <= MAX_BINS]
+    if not cands:
+        raise ValueError("no candidate fits the Morton-bin limit (n_meshes << bits <= 2^30)")
     forest.reserve(n, max(c[0] for c in cands), dev)
     dist = torch.empty(n, dtype=torch.float32, device=dev)
     face = torch.empty(n, dtype=torch.int64, device=dev)
     out = (dist, face, None)
     times = {}
-    for cfg in cands:
+
+    def measure(cfg):
         forest.unsigned_distance(points, offsets, False, cfg, out)  # warm up
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()
@@ -83,7 +94,17 @@ def tune_forest(forest, points, offsets, candidates=None, reps=3, use_cache=True
         ev[1].record()
         torch.cuda.synchronize(dev)
         times[cfg] = ev[0].elapsed_time(ev[1]) / reps
+
+    for cfg in cands:
+        measure(cfg)
     best = min(times, key=times.get)
+    if candidates is None:
+        # second stage: the workgroup size of the best launch
+        for wpb in WAVES_PER_BLOCK:
+            cfg = best[:3] + (wpb,)
+            if cfg not in times:
+                measure(cfg)
+        best = min(times, key=times.get)
     ms = {",".join(map(str, k)): v for k, v in times.items()}
     if use_cache:
         cache[key] = {"best": list(best), "ms": ms}

@@ -361,10 +361,12 @@ int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* 
         rc = ensure_pool(device);
         if (rc != CUBVH_OK) return rc;
         const size_t ob = ((size_t)n * 4 + 255) & ~(size_t)255;
-        hipError_t e = hipMallocAsync(&scratch, ob + morton_workspace_bytes(n), hs);
+        // raystab: the stab passes' lists after the order, the ordering workspace after them
+        const size_t sb = kmode == 2 ? ((stab_workspace_bytes(n) + 255) & ~(size_t)255) : 0;
+        hipError_t e = hipMallocAsync(&scratch, ob + sb + morton_workspace_bytes(n), hs);
         if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(query order)");
         uint32_t* order = (uint32_t*)scratch;
-        e = morton_order(n, positions, (char*)scratch + ob, order, hs);
+        e = morton_order(n, positions, (char*)scratch + ob + sb, order, hs);
         if (e != hipSuccess) {
             (void)hipFreeAsync(scratch, hs);
             return fail_hip(e, "query ordering");
@@ -376,7 +378,7 @@ int run_distance(cubvh_bvh* bvh, int device, int64_t n, int kmode, const float* 
     if (kmode == 2 && q.packet) {
         // raystab over Morton-ordered queries: packet distance walk, then the stab rays
         e = launch_distance(q, 0);
-        if (e == hipSuccess) e = launch_stab_sign(q);
+        if (e == hipSuccess) e = launch_stab_sign(q, (char*)scratch + (((size_t)n * 4 + 255) & ~(size_t)255));
     } else {
         e = launch_distance(q, kmode);
     }
@@ -889,10 +891,27 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
     if (!grad_dist || !positions || !face_id || !uvw || !vertices || !faces)
         return fail(CUBVH_ERR_ARG, "null input buffer");
     if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "empty mesh");
+    // the vertex gradients sort queries by face and face corners by vertex (int counts)
+    if (n >= ((int64_t)1 << 31)) return fail(CUBVH_ERR_ARG, "too many queries (at most 2^31 - 1 per call)");
+    if (n_vertices >= ((int64_t)1 << 31) || 3 * n_faces >= ((int64_t)1 << 31))
+        return fail(CUBVH_ERR_ARG, "mesh too large for the backward (vertices and 3 x faces below 2^31)");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
     DeviceScope g(device);
     if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    hipStream_t hs = (hipStream_t)stream;
+    void* ws = nullptr;
+    if (grad_vertices) {
+        int rc = ensure_pool(device);
+        if (rc != CUBVH_OK) return rc;
+        hipError_t e = hipMallocAsync(&ws, udf_backward_workspace_bytes(n, n_vertices, n_faces, grad_points != nullptr), hs);
+        if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(backward workspace)");
+    }
     hipError_t e = launch_udf_backward(n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
-                                       grad_points, grad_vertices, (hipStream_t)stream);
+                                       grad_points, grad_vertices, ws, hs);
+    if (ws) {
+        hipError_t e2 = hipFreeAsync(ws, hs);
+        if (e == hipSuccess) e = e2;
+    }
     if (e != hipSuccess) return fail_hip(e, "backward kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END
@@ -1062,17 +1081,21 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
 
 int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,
                                    const int64_t* query_offsets, float* distances, int64_t* face_id, float* uvw,
-                                   const int32_t* tuning, void* stream) {
+                                   const int32_t* tuning, int32_t n_tuning, void* stream) {
     CUBVH_GUARD_BEGIN
     if (!forest) return fail(CUBVH_ERR_ARG, "forest is null");
+    if (n_tuning < 0 || n_tuning > 4 || (n_tuning > 0 && !tuning))
+        return fail(CUBVH_ERR_ARG, "tuning must hold 0 to 4 entries");
     if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
     if (n < 0 || n >= ((int64_t)1 << 32) - 64 * (int64_t)forest->n_meshes)
         return fail(CUBVH_ERR_ARG, "query count out of range");
     if (n == 0) return CUBVH_OK;
     if (!positions || !query_offsets || !distances || !face_id) return fail(CUBVH_ERR_ARG, "null query buffer");
-    const int32_t bits = tuning ? tuning[0] : 12;
-    const int32_t persistent = tuning ? tuning[1] : 1;
-    const int32_t wpc = tuning ? tuning[2] : 8;
+    const int32_t bits = n_tuning > 0 ? tuning[0] : 12;
+    const int32_t persistent = n_tuning > 1 ? tuning[1] : 1;
+    const int32_t wpc = n_tuning > 2 ? tuning[2] : 8;
+    const int32_t wpb = n_tuning > 3 ? tuning[3] : 4;
+    if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(CUBVH_ERR_ARG, "waves per block must be 1, 2, 4 or 8");
     if (bits < 0 || bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
     // a bin index is mesh << bits | code in 32 bits; the histogram holds one word per bin
     if (((int64_t)forest->n_meshes << bits) > kMaxForestBins)
@@ -1111,6 +1134,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.mesh_info = c.mesh_info;
     a.persistent = persistent;
     a.waves_per_cu = wpc;
+    a.waves_per_block = wpb;
     a.n_cu = c.n_cu;
     e = launch_forest_distance(a, ws, hs);
     if (e != hipSuccess) return fail_hip(e, "forest kernel launch");

@@ -42,6 +42,7 @@ struct ForestArgs {
     const int4* mesh_info = nullptr;  // per mesh: root node, face-0 slot, n_faces, 0
     int32_t persistent = 0;
     int32_t waves_per_cu = 8;
+    int32_t waves_per_block = 4;  // 1, 2, 4 or 8 (workgroups of 64..512 lanes)
     int32_t n_cu = 256;
 };
 

@@ -840,34 +840,89 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
     }
 }
 
-// Raystab sign of distances already computed (src/bvh.cu:408-427): lane i takes query
-// order[i] (or i) and negates distances[q] when all 64 stab rays from its point hit a
-// face.  The stab directions are keyed by the query index q, as in distance_kernel.
+// Raystab sign of distances already computed (src/bvh.cu:408-427): a query is inside,
+// and its distance negated, when all 64 stab rays from its point hit a face (the 32
+// stab lines, each hit both ways).  The stab directions are keyed by the query index,
+// as in distance_kernel.  Most queries are decided by their first line, a few need all
+// 32, so the lines are spread over the lanes in two launches instead of one lane
+// walking every line of its query:
+//   stab_first : every query (in the Morton order) walks its line 0 and stores its two
+//                random offsets; the queries whose line hits both ways are appended to
+//                `list` (counts[0]) with their flag cleared;
+//   stab_rest  : one lane per (listed query, line 1..31), line-major, so that lines
+//                of a query already found outside (flag set) are skipped;
+//   stab_negate: the listed queries whose flag stayed clear are inside.
+// The decisions are the ones of walking a query's lines in order up to the first miss.
+__global__ void stab_init(uint32_t* counts) {
+    if (threadIdx.x == 0) counts[0] = 0u;
+}
+
 template <int BLOCK, int LDS_N>
-__global__ __launch_bounds__(BLOCK) void stab_sign_kernel(int64_t n, const float* __restrict__ positions,
-                                                          const uint32_t* __restrict__ order,
-                                                          float* __restrict__ distances, Tree T) {
+__global__ __launch_bounds__(BLOCK) void stab_first(int64_t n, const uint32_t* __restrict__ order,
+                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
+                                                    uint32_t* __restrict__ flags, const float* __restrict__ positions,
+                                                    float2* __restrict__ offs, Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
-    const int64_t gi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (gi >= n) return;
-    const int64_t q = order ? (int64_t)order[gi] : gi;
-    if (q >= n) return;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
     st.tid = (int)threadIdx.x;
     st.sp = 0;
-    const f3 p = load_pt(positions, q);
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t q = order[i];
     Pcg32 rng;
-    rng.advance((uint64_t)(uint32_t)((uint32_t)q * 2u));
-    float ox = rng.next_float();
-    float oy = rng.next_float();
-    for (uint32_t k = 0; k < 32; ++k) {
-        f3 dir = fibonacci_dir32(k, ox, oy);
-        if (any_hit_line(T, p, dir, st) != 3) return;  // no hit along +dir or -dir: outside
+    rng.advance((uint64_t)(uint32_t)(q * 2u));
+    float2 o;
+    o.x = rng.next_float();
+    o.y = rng.next_float();
+    offs[q] = o;
+    const bool keep = any_hit_line(T, load_pt(positions, q), fibonacci_dir32(0u, o.x, o.y), st) == 3;
+    const uint64_t m = ballot(keep);
+    if (m == 0) return;
+    const int leader = (int)__builtin_ctzll(m);
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(&counts[0], (uint32_t)__builtin_popcountll(m));
+    b = (uint32_t)__builtin_amdgcn_readlane((int32_t)b, leader);
+    if (keep) {
+        list[b + lanes_below(m)] = q;
+        flags[q] = 0u;
     }
-    distances[q] = -distances[q];
+}
+
+template <int BLOCK, int LDS_N>
+__global__ __launch_bounds__(BLOCK) void stab_rest(const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ counts, uint32_t* __restrict__ flags,
+                                                   const float* __restrict__ positions,
+                                                   const float2* __restrict__ offs, Tree T) {
+    __shared__ int32_t s_ref[LDS_N * BLOCK];
+    __shared__ float s_key[LDS_N * BLOCK];
+    LaneStack<BLOCK, LDS_N> st;
+    st.ref = s_ref;
+    st.key = s_key;
+    st.tid = (int)threadIdx.x;
+    st.sp = 0;
+    const int64_t m = (int64_t)counts[0];
+    const int64_t work = m * 31;  // item t: line 1 + t / m of query list[t % m]
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < work; t += stride) {
+        const uint32_t k = 1u + (uint32_t)(t / m);
+        const uint32_t q = list[t % m];
+        if (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) continue;
+        const float2 o = offs[q];
+        if (any_hit_line(T, load_pt(positions, q), fibonacci_dir32(k, o.x, o.y), st) != 3)
+            atomicOr(&flags[q], 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void stab_negate(const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ flags, float* __restrict__ distances) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)counts[0]) return;
+    const uint32_t q = list[i];
+    if (flags[q] == 0u) distances[q] = -distances[q];
 }
 
 constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal search
@@ -927,8 +982,8 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
 // Unsigned distance over a forest: one wave per 64-slot tile of the padded query
 // order; every tile belongs to one mesh, whose root node and face-0 slot come from
 // mesh_info.  PERSISTENT waves take tiles from an atomic counter until none is left.
-template <int PERSISTENT>
-__global__ __launch_bounds__(256) void forest_distance_kernel(const float* __restrict__ positions,
+template <int PERSISTENT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void forest_distance_kernel(const float* __restrict__ positions,
                                                               const uint32_t* __restrict__ order,
                                                               const int64_t* __restrict__ pbase, int32_t n_meshes,
                                                               const int4* __restrict__ mesh_info,
@@ -936,12 +991,12 @@ __global__ __launch_bounds__(256) void forest_distance_kernel(const float* __res
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
                                                               TreeArrays arrays, uint32_t* counter,
                                                               int64_t max_tiles) {
-    __shared__ ExactQueue queues[4];
+    __shared__ ExactQueue queues[WAVES];
     const int lane = (int)(threadIdx.x & 63);
     // max_tiles: the tiles the order array holds (bounds the walk even when the offsets
     // are not a CSR of the queries)
     const int64_t n_tiles = min(pbase[n_meshes] / 64, max_tiles);
-    int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
+    int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6));
     for (;;) {
         if (PERSISTENT) {
             uint32_t t = 0;
@@ -1103,21 +1158,36 @@ hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
 
 namespace cubvh_amd {
 
-hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
+namespace {
+
+template <int WAVES>
+hipError_t launch_forest_shape(const ForestArgs& a, const ForestWorkspace& ws, hipStream_t s) {
+    const int64_t max_tiles = ws.npad_max / 64;
     if (a.persistent) {
-        const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / 4);
-        hipLaunchKernelGGL((dev::forest_distance_kernel<1>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter,
-                           ws.npad_max / 64);
+        const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / WAVES);
+        hipLaunchKernelGGL((dev::forest_distance_kernel<1, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, a.positions,
+                           ws.order, ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree,
+                           ws.counter, max_tiles);
     } else {
-        const int64_t max_tiles = ws.npad_max / 64;
-        const unsigned blocks = (unsigned)((max_tiles + 3) / 4);
-        hipLaunchKernelGGL((dev::forest_distance_kernel<0>), dim3(blocks), dim3(256), 0, s, a.positions, ws.order,
-                           ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree, ws.counter,
-                           max_tiles);
+        const unsigned blocks = (unsigned)((max_tiles + WAVES - 1) / WAVES);
+        hipLaunchKernelGGL((dev::forest_distance_kernel<0, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, a.positions,
+                           ws.order, ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree,
+                           ws.counter, max_tiles);
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_forest_distance(const ForestArgs& a, const ForestWorkspace& ws, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    switch (a.waves_per_block) {
+        case 1: return launch_forest_shape<1>(a, ws, s);
+        case 2: return launch_forest_shape<2>(a, ws, s);
+        case 4: return launch_forest_shape<4>(a, ws, s);
+        case 8: return launch_forest_shape<8>(a, ws, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 int stack_capacity_for(int need) { return need <= dev::STACK_TOTAL ? dev::STACK_TOTAL : -1; }
@@ -1128,13 +1198,40 @@ hipError_t launch_distance(const QueryArgs& q, int mode) {
     return launch_distance_impl(q, mode);
 }
 
-hipError_t launch_stab_sign(const QueryArgs& q) {
+size_t stab_workspace_bytes(int64_t n) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return al(64) + 2 * al(4 * (size_t)n) + al(8 * (size_t)n);
+}
+
+hipError_t launch_stab_sign(const QueryArgs& q, void* ws) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
+    if (!q.order || !ws || q.n >= ((int64_t)1 << 32)) return hipErrorInvalidValue;
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
-    const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((dev::stab_sign_kernel<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0,
-                       (hipStream_t)q.stream, q.n, q.in0, q.order, q.out_f, T);
+    hipStream_t s = (hipStream_t)q.stream;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char* w = (char*)ws;
+    uint32_t* counts = (uint32_t*)w;
+    w += al(64);
+    uint32_t* list = (uint32_t*)w;
+    w += al(4 * (size_t)q.n);
+    uint32_t* flags = (uint32_t*)w;
+    w += al(4 * (size_t)q.n);
+    float2* offs = (float2*)w;
+    int dev_id = 0, n_cu = 256;
+    (void)hipGetDevice(&dev_id);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev_id);
+    const int64_t full = (q.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts);
+    hipLaunchKernelGGL((dev::stab_first<kBlock, kLdsEntries>), dim3((unsigned)full), dim3(kBlock), 0, s, q.n, q.order,
+                       list, counts, flags, q.in0, offs, T);
+    // lines 1..31 of the listed queries: a grid that fills the device, striding over
+    // (list length x 31) items read on the device
+    const unsigned rest = (unsigned)std::min<int64_t>(full * 31, (int64_t)n_cu * 32);
+    hipLaunchKernelGGL((dev::stab_rest<kBlock, kLdsEntries>), dim3(rest), dim3(kBlock), 0, s, list, counts, flags, q.in0,
+                       offs, T);
+    hipLaunchKernelGGL(dev::stab_negate, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s, list, counts, flags,
+                       q.out_f);
     return hipGetLastError();
 }
 

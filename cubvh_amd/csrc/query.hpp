@@ -39,8 +39,11 @@ int stack_capacity_for(int need);
 // mode 0 unsigned distance, 1 signed (watertight), 2 signed (raystab)
 hipError_t launch_distance(const QueryArgs& q, int mode);
 hipError_t launch_ray(const QueryArgs& q);
-// Negates q.out_f[i] of every query whose 64 raystab rays all hit (distances from mode 0).
-hipError_t launch_stab_sign(const QueryArgs& q);
+// Negates q.out_f[i] of every query whose 64 raystab rays all hit (distances from mode 0),
+// walking the queries of q.order (required, n < 2^32) in passes over the undecided ones;
+// ws holds stab_workspace_bytes(q.n) bytes.
+size_t stab_workspace_bytes(int64_t n);
+hipError_t launch_stab_sign(const QueryArgs& q, void* ws);
 
 // Morton permutation of point queries (order.hip).
 size_t morton_workspace_bytes(int64_t n);
@@ -57,8 +60,13 @@ hipError_t noalloc_scratch(int device, void** ptr);
 hipError_t launch_floodfill(const uint8_t* grid, int32_t B, int32_t H, int32_t W, int32_t D, int32_t* labels,
                             hipStream_t stream);
 
+// Backward of the unsigned distance (grad.hip).  With grad_vertices, `ws` holds
+// udf_backward_workspace_bytes(...) bytes of device scratch (sorts and per-face sums);
+// point gradients alone need none.
+size_t udf_backward_workspace_bytes(int64_t n, int64_t n_vertices, int64_t n_faces, bool have_grad_points);
 hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* positions, const int64_t* face_id,
                                const float* uvw, const float* vertices, int64_t n_vertices, const int32_t* faces,
-                               int64_t n_faces, float* grad_points, float* grad_vertices, hipStream_t stream);
+                               int64_t n_faces, float* grad_points, float* grad_vertices, void* ws,
+                               hipStream_t stream);
 
 }  // namespace cubvh_amd

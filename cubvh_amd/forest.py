@@ -19,7 +19,8 @@ import torch
 from . import _lib
 from .api import cuBVH, _query_tensor, _stream_of
 
-DEFAULT_TUNING = (12, 1, 8)  # Morton-prefix bits, persistent grid, waves per CU
+# Morton-prefix bits, persistent grid, waves per CU, waves per workgroup
+DEFAULT_TUNING = (12, 1, 8, 4)
 
 
 def _offsets_tensor(offsets, n, device, check=True):
@@ -97,11 +98,13 @@ class cuBVHForest:
             uvw = torch.empty(n, 3, dtype=torch.float32, device=dev) if return_uvw else None
         else:
             dist, face, uvw = out
-        t = tuning if tuning is not None else DEFAULT_TUNING
-        tarr = (ctypes.c_int32 * 3)(int(t[0]), int(t[1]), int(t[2]))
+        t = tuple(tuning) if tuning is not None else DEFAULT_TUNING
+        if not 1 <= len(t) <= 4:
+            raise ValueError("tuning holds 1 to 4 entries (bits, persistent, waves per CU, waves per workgroup)")
+        tarr = (ctypes.c_int32 * len(t))(*[int(x) for x in t])
         _lib.check(self._lib.cubvh_forest_unsigned_distance(
             self._h, dev.index, n, points.data_ptr(), offsets.data_ptr(), dist.data_ptr(), face.data_ptr(),
-            uvw.data_ptr() if uvw is not None else None, tarr, _stream_of(points)))
+            uvw.data_ptr() if uvw is not None else None, tarr, len(t), _stream_of(points)))
         return dist, face, uvw
 
     # ------------------------------------------------------------- helpers

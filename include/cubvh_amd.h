@@ -184,7 +184,12 @@ int cubvh_floodfill(int device, const uint8_t* grid, int32_t B, int32_t H, int32
  *   grad_vertices[v] += -grad_dist[i] * uvw_i[k] * n_i  for the k-th vertex v of the face
  * vertices fp32 [n_vertices,3], faces int32 [n_faces,3] (original mesh, on device);
  * grad_points fp32 [n,3] may be NULL; grad_vertices fp32 [n_vertices,3] may be NULL and
- * is accumulated into (the caller zeroes it). */
+ * is accumulated into (the caller zeroes it).  The vertex sums are gathered in a fixed
+ * order without atomics (per face its queries by ascending index, summed in chunks of 32
+ * that are then added in order; per vertex its corners 3 f + k ascending): the same bits
+ * on every call.  With grad_vertices the call
+ * allocates stream-ordered scratch (sorts of the n face ids and 3 n_faces corners);
+ * n < 2^31 and 3 n_faces < 2^31. */
 int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,
                        const int64_t* face_id, const float* uvw, const float* vertices, int64_t n_vertices,
                        const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
@@ -224,14 +229,16 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
  * (offsets[0] = 0, non-decreasing, offsets[n_meshes] = n); they are read on the device
  * only, so other offsets are not reported: they give wrong answers, but every write
  * stays inside the outputs and the workspace.  Outputs as cubvh_unsigned_distance.
- * tuning (nullable = {12, 1, 8}): [0] Morton-prefix bits used to order the queries
- * inside each mesh (0..16, 0 keeps input order; n_meshes * 2^bits <= 2^30), [1] 1 = persistent grid taking
- * 64-query tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of
- * the persistent grid (1..32).  Grows the workspace when the reserved one is too
- * small, which is refused while the stream is being captured. */
+ * tuning: n_tuning (0..4) entries, the missing ones default to {12, 1, 8, 4}:
+ * [0] Morton-prefix bits used to order the queries inside each mesh (0..16, 0 keeps
+ * input order; n_meshes * 2^bits <= 2^30), [1] 1 = persistent grid taking 64-query
+ * tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of the
+ * persistent grid (1..32), [3] waves per workgroup (1, 2, 4 or 8).  Grows the workspace
+ * when the reserved one is too small, which is refused while the stream is being
+ * captured. */
 int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,
                                    const int64_t* query_offsets, float* distances, int64_t* face_id, float* uvw,
-                                   const int32_t* tuning, void* stream);
+                                   const int32_t* tuning, int32_t n_tuning, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Sparse-voxel hash table: a static open-addressed table of int32 keys of D words

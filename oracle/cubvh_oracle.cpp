@@ -71,6 +71,15 @@
 //      compiled from its header (same triangles and vertex positions, vertex order
 //      aside) and against closed-surface invariants.
 //
+//  (9) backward of the unsigned distance (no counterpart in the reference): the
+//      envelope-theorem gradient (grad_p = g n, grad_v_k -= w_k g n with
+//      n = (p - c) / |p - c|, c rebuilt from face + uvw) in fp32 with the product's
+//      operations and summation order -- per face its queries in ascending index, in
+//      chunks of 32 summed separately and then added in order; per vertex its corners
+//      3 f + k in ascending order -- so that the product's
+//      vertex gradients can be compared bit for bit.  Pinned against FP64 central
+//      finite differences of the FP64 brute-force distance (tests/test_oracle_backward.py).
+//
 // sin/cos of the stab directions are evaluated as (float)sin((double)x): a libm
 // result, not the CUDA math library.
 
@@ -1551,6 +1560,70 @@ int orc_smc(const int32_t* coords, const float* corners, int64_t n, float iso, i
     }
     *n_verts = m;
     *n_tris = t_out;
+    return 0;
+}
+
+// Backward of sum_i g_i d_i (see (9) above).  gp [n,3] is written; gv [nV,3] is
+// added to.  Queries with an out-of-range face or vertex, or with d = 0 or g = 0,
+// add nothing to gv (gp is g n, zero for the out-of-range ones).
+int orc_udf_backward(const float* V, int64_t nV, const int32_t* F, int64_t nF, const float* p, const float* g,
+                     const int64_t* face, const float* uvw, int64_t n, float* gp, float* gv) {
+    std::vector<float> gn(3 * (size_t)n, 0.0f);
+    std::vector<std::vector<int64_t>> by_face((size_t)nF);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t f = face[i];
+        bool ok = f >= 0 && f < nF;
+        const int32_t ia = ok ? F[3 * f] : 0, ib = ok ? F[3 * f + 1] : 0, ic = ok ? F[3 * f + 2] : 0;
+        ok = ok && ia >= 0 && ia < nV && ib >= 0 && ib < nV && ic >= 0 && ic < nV;
+        float s = 0.0f;
+        if (ok) {
+            const float u = uvw[3 * i], v = uvw[3 * i + 1], w = uvw[3 * i + 2];
+            float d[3];
+            for (int k = 0; k < 3; ++k) {
+                const float c = std::fmaf(w, V[3 * ic + k], std::fmaf(v, V[3 * ib + k], u * V[3 * ia + k]));
+                d[k] = p[3 * i + k] - c;
+            }
+            const float len = std::sqrt(std::fmaf(d[2], d[2], std::fmaf(d[1], d[1], d[0] * d[0])));
+            s = len > 0.0f ? g[i] / len : 0.0f;
+            for (int k = 0; k < 3; ++k) gn[3 * i + k] = d[k] * s;
+        }
+        for (int k = 0; k < 3; ++k) gp[3 * i + k] = gn[3 * i + k];
+        if (ok && s != 0.0f) by_face[(size_t)f].push_back(i);
+    }
+    // per face: its queries in chunks of 32 from the first, each chunk summed in order,
+    // then the chunk sums added in order
+    const size_t kChunk = 32;
+    std::vector<float> acc(9 * (size_t)nF, 0.0f);
+    for (int64_t f = 0; f < nF; ++f) {
+        const std::vector<int64_t>& qs = by_face[(size_t)f];
+        float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t c0 = 0; c0 < qs.size(); c0 += kChunk) {
+            float part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (size_t j = c0; j < std::min(qs.size(), c0 + kChunk); ++j) {
+                const int64_t q = qs[j];
+                for (int k = 0; k < 3; ++k) {
+                    const float wk = uvw[3 * q + k];
+                    for (int c = 0; c < 3; ++c) part[3 * k + c] += -wk * gn[3 * q + c];
+                }
+            }
+            for (int k = 0; k < 9; ++k) a[k] += part[k];
+        }
+        for (int k = 0; k < 9; ++k) acc[9 * f + k] = a[k];
+    }
+    std::vector<std::vector<int64_t>> by_vertex((size_t)nV);
+    for (int64_t s = 0; s < 3 * nF; ++s)
+        if (F[s] >= 0 && F[s] < nV) by_vertex[(size_t)F[s]].push_back(s);
+    for (int64_t v = 0; v < nV; ++v) {
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        for (int64_t s : by_vertex[(size_t)v]) {
+            x += acc[3 * s];
+            y += acc[3 * s + 1];
+            z += acc[3 * s + 2];
+        }
+        gv[3 * v] += x;
+        gv[3 * v + 1] += y;
+        gv[3 * v + 2] += z;
+    }
     return 0;
 }
 

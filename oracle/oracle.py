@@ -76,6 +76,7 @@ def load():
         "orc_udf_f64": (c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp, vp, c_int]),
         "orc_ray_f64": (c_int, [vp, i64, vp, i64, vp, vp, i64, c_double, vp, vp, vp, c_int]),
         "orc_winding_f64": (c_int, [vp, i64, vp, i64, vp, i64, vp, c_int]),
+        "orc_udf_backward": (c_int, [vp, i64, vp, i64, vp, vp, vp, vp, i64, vp, vp]),
         "orc_refbvh_build": (vp, [vp, i64, vp, i64]),
         "orc_refbvh_import": (vp, [vp, i64, vp, i64]),
         "orc_refbvh_free": (None, [vp]),
@@ -216,6 +217,21 @@ def ray_f64(V, F, ro, rd, tol=1e-6, nthreads=0):
                             nthreads)
     assert rc == 0
     return t, f, amb.astype(bool)
+
+
+def udf_backward(V, F, pts, g, face, uvw):
+    """Gradients of sum_i g_i d_i (cubvh_oracle.cpp (9)): grad_points [n,3] and
+    grad_vertices [V,3] in fp32, with the product's operations and summation order."""
+    V, F, pts = _f32(V), _i32(F), _f32(pts).reshape(-1, 3)
+    g = _f32(g).reshape(-1)
+    face = np.ascontiguousarray(face, dtype=np.int64).reshape(-1)
+    uvw = _f32(uvw).reshape(-1, 3)
+    n = len(pts)
+    gp = np.empty((n, 3), np.float32)
+    gv = np.zeros((len(V), 3), np.float32)
+    rc = load().orc_udf_backward(_p(V), len(V), _p(F), len(F), _p(pts), _p(g), _p(face), _p(uvw), n, _p(gp), _p(gv))
+    assert rc == 0
+    return gp, gv
 
 
 def winding_f64(V, F, pts, nthreads=0):

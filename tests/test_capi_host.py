@@ -284,8 +284,15 @@ def test_forest_morton_bins_limit_refused_before_device_work():
         assert lib.cubvh_forest_reserve(h, 0, 1000, 16, None) == -1
         assert b"Morton bins" in lib.cubvh_last_error()
         tun = (ctypes.c_int32 * 4)(16, 1, 8, 4)
-        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, None) == -1
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, 4, None) == -1
         assert b"Morton bins" in lib.cubvh_last_error()
+        # the tuning array's length is checked, and so is the workgroup shape
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, 5, None) == -1
+        assert b"0 to 4 entries" in lib.cubvh_last_error()
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, None, 2, None) == -1
+        bad = (ctypes.c_int32 * 4)(0, 1, 8, 3)
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, bad, 4, None) == -1
+        assert b"waves per block" in lib.cubvh_last_error()
     finally:
         lib.cubvh_forest_destroy(h)
 

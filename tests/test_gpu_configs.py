@@ -293,8 +293,9 @@ def test_forest512_equals_per_mesh_and_oracle(gpu, oracle, forest512, record):
 
 
 def test_forest512_backward(gpu, forest512, record):
-    """Batched fwd + bwd on all 512 meshes: point gradients equal the per-mesh autograd
-    path bit for bit; vertex gradients (fp32 atomics, order-dependent) within 1e-5."""
+    """Batched fwd + bwd on all 512 meshes: point and vertex gradients equal the
+    per-mesh autograd path bit for bit (a mesh's vertex gradients are summed over its own
+    queries in their order, whatever else shares the call)."""
     specs, meshes, bvhs, pts, offsets = forest512
     forest = cubvh_amd.cuBVHForest(bvhs)
     P = torch.from_numpy(pts).to(gpu).requires_grad_(True)
@@ -312,6 +313,6 @@ def test_forest512_backward(gpu, forest512, record):
         (dd * dd).sum().backward()
         assert torch.equal(P.grad[sl], p.grad), m
         gv = V.grad[int(vbase[m]):int(vbase[m + 1])]
-        torch.testing.assert_close(gv, v.grad, rtol=1e-5, atol=1e-5)
+        assert torch.equal(gv, v.grad), m
         worst = max(worst, float((gv - v.grad).abs().max()))
     record("forest512_backward", meshes_checked=8, max_abs_vertex_grad_diff=worst)
