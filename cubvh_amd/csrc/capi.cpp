@@ -858,8 +858,29 @@ int cubvh_ray_trace(cubvh_bvh* bvh, int device, int64_t n, const float* rays_o, 
     q.out_v = positions;
     q.out_id = face_id;
     q.out_f = depth;
-    q.packet = n >= kMortonMin;  // packets from 2,048 rays on, in the caller's order
+    // packets from 2,048 rays on, walked in an order that keeps rays of close origins and
+    // directions together (stream-ordered scratch, released after the launch)
+    q.packet = n >= kMortonMin;
+    hipStream_t hs = (hipStream_t)stream;
+    void* scratch = nullptr;
+    if (q.packet && n < ((int64_t)1 << 31)) {
+        rc = ensure_pool(device);
+        if (rc != CUBVH_OK) return rc;
+        const size_t ob = ((size_t)n * 4 + 255) & ~(size_t)255;
+        hipError_t e = hipMallocAsync(&scratch, ob + morton_workspace_bytes(n), hs);
+        if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(ray order)");
+        e = ray_order(n, rays_o, rays_d, (char*)scratch + ob, (uint32_t*)scratch, hs);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(scratch, hs);
+            return fail_hip(e, "ray ordering");
+        }
+        q.order = (const uint32_t*)scratch;
+    }
     hipError_t e = launch_ray(q);
+    if (scratch) {
+        hipError_t e2 = hipFreeAsync(scratch, hs);
+        if (e == hipSuccess) e = e2;
+    }
     if (e != hipSuccess) return fail_hip(e, "ray kernel launch");
     return CUBVH_OK;
     CUBVH_GUARD_END

@@ -107,6 +107,54 @@ __global__ __launch_bounds__(256) void morton_kernel(int64_t n, const float* __r
     idx[i] = (uint32_t)i;
 }
 
+// Two 9-bit values interleaved (x in the even bits).
+__device__ __forceinline__ uint32_t spread9x2(uint32_t x) {
+    x &= 0x1ffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// Ray sort key: the Morton code of the origin's cell in the origins' bounds (4 bits per
+// axis) above the Morton code of the direction's cell on the octahedral map of the unit
+// sphere (9 bits per axis).  Rays of one origin cell and similar directions -- e.g. a
+// small square of pixels of one camera -- end up next to each other.
+__global__ __launch_bounds__(256) void ray_key_kernel(int64_t n, const float* __restrict__ o,
+                                                      const float* __restrict__ d, const int32_t* __restrict__ b,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = unordered(b[k]), hi = unordered(b[3 + k]);
+        float ext = hi - lo;
+        float sc = ext > 0.0f ? 15.0f / ext : 0.0f;
+        float t = fminf(fmaxf((o[3 * i + k] - lo) * sc, 0.0f), 15.0f);  // NaN maps to 0
+        c[k] = (uint32_t)t;
+    }
+    const uint32_t ko = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
+    const float dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+    const float sum = fabsf(dx) + fabsf(dy) + fabsf(dz);
+    float px = 0.0f, py = 0.0f;
+    if (sum > 0.0f && sum < INFINITY) {
+        px = dx / sum;
+        py = dy / sum;
+        if (dz < 0.0f) {
+            const float tx = (1.0f - fabsf(py)) * copysignf(1.0f, px);
+            const float ty = (1.0f - fabsf(px)) * copysignf(1.0f, py);
+            px = tx;
+            py = ty;
+        }
+    }
+    const uint32_t u = (uint32_t)fminf(fmaxf((px + 1.0f) * 256.0f, 0.0f), 511.0f);
+    const uint32_t v = (uint32_t)fminf(fmaxf((py + 1.0f) * 256.0f, 0.0f), 511.0f);
+    keys[i] = (ko << 18) | (spread9x2(u) << 1) | spread9x2(v);
+    idx[i] = (uint32_t)i;
+}
+
 }  // namespace dev
 
 hipError_t launch_bounds(int64_t n, const float* positions, int32_t* bounds, hipStream_t s) {
@@ -127,6 +175,32 @@ size_t morton_workspace_bytes(int64_t n) {
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     return al(64) + 3 * al(4 * (size_t)n) + al(tmp);
+}
+
+// Writes the permutation of n rays sorted by ray_key_kernel's key into order_out, using
+// `ws` (morton_workspace_bytes(n) bytes: the same layout as the point ordering).
+// n must be < 2^31.
+hipError_t ray_order(int64_t n, const float* rays_o, const float* rays_d, void* ws, uint32_t* order_out, hipStream_t s) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char* w = (char*)ws;
+    int32_t* bounds = (int32_t*)w;
+    w += al(64);
+    uint32_t* k0 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    uint32_t* k1 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    uint32_t* v0 = (uint32_t*)w;
+    w += al(4 * (size_t)n);
+    size_t tmp = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    if (e != hipSuccess) return e;
+    e = launch_bounds(n, rays_o, bounds, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::ray_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, rays_o, rays_d, bounds,
+                       k0, v0);
+    e = hipcub::DeviceRadixSort::SortPairs((void*)w, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
 }
 
 // Scratch of the allocation-free distance entry point (cubvh_unsigned_distance_noalloc):

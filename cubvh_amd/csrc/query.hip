@@ -1068,12 +1068,15 @@ __global__ __launch_bounds__(BLOCK) void ray_kernel(int64_t n, const float* __re
 constexpr int kRayLds = 8;
 __global__ __launch_bounds__(256) void ray_packet_kernel(int64_t n, const float* __restrict__ rays_o,
                                                          const float* __restrict__ rays_d,
+                                                         const uint32_t* __restrict__ order,
                                                          float* __restrict__ positions, int64_t* __restrict__ face_id,
                                                          float* __restrict__ depth, Tree T) {
     __shared__ int32_t s_ref[kRayLds * 256];
     __shared__ float s_key[kRayLds * 256];
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t i = g;  // ray index (lane g takes ray order[g] when an order is given)
+    if (g < n && order) i = (int64_t)order[g];
+    const bool valid = g < n && i < n;
     f3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(1.0f, 0.0f, 0.0f);
     if (valid) {
         o = load_pt(rays_o, i);
@@ -1242,7 +1245,7 @@ hipError_t launch_ray(const QueryArgs& q) {
     if (q.packet) {
         const unsigned grid = (unsigned)((q.n + 255) / 256);
         hipLaunchKernelGGL(dev::ray_packet_kernel, dim3(grid), dim3(256), 0, (hipStream_t)q.stream, q.n, q.in0, q.in1,
-                           q.out_v, q.out_id, q.out_f, T);
+                           q.order, q.out_v, q.out_id, q.out_f, T);
         return hipGetLastError();
     }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);

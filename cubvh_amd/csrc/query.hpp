@@ -48,6 +48,9 @@ hipError_t launch_stab_sign(const QueryArgs& q, void* ws);
 // Morton permutation of point queries (order.hip).
 size_t morton_workspace_bytes(int64_t n);
 hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* order_out, hipStream_t s);
+// Permutation of rays that puts rays of close origins and directions together (order.hip);
+// ws holds morton_workspace_bytes(n) bytes.
+hipError_t ray_order(int64_t n, const float* rays_o, const float* rays_d, void* ws, uint32_t* order_out, hipStream_t s);
 
 // Static per-device scratch of the allocation-free distance call (order.hip): room for
 // the Morton order and ordering workspace of kNoAllocMaxQueries queries.
