@@ -4,7 +4,9 @@ Tries the launch configurations of :meth:`cuBVHForest.unsigned_distance` on the
 caller's own batch -- Morton-prefix bits used to order the queries inside each mesh
 (the query-sort granularity), persistent grid or one wave per tile, waves per CU of
 the persistent grid, then waves per workgroup (the block size: 64 to 512 lanes) for
-the best of those -- times each with HIP events and caches the winner per (device,
+the best of those, then for a one-wave-per-tile best the runs of consecutive
+workgroups dealt to each XCD (so that neighbouring tiles share an L2) -- times each
+with HIP events and caches the winner per (device,
 query-count bucket, mesh-count bucket, face-count bucket) in a JSON file
 (``$CUBVH_AMD_TUNE_CACHE`` or ``~/.cache/cubvh_amd/autotune.json``).
 
@@ -22,11 +24,12 @@ from pathlib import Path
 import torch
 
 CANDIDATES = [
-    (bits, persistent, wpc, 4)
+    (bits, persistent, wpc, 4, 0)
     for bits in (0, 6, 9, 12, 15)
     for persistent, wpc in ((0, 8), (1, 4), (1, 8), (1, 16))
 ]
 WAVES_PER_BLOCK = (1, 2, 4, 8)
+XCD_CHUNKS = (0, 4, 16, 64)
 MAX_BINS = 1 << 30  # n_meshes << bits (the library refuses more Morton bins)
 
 
@@ -101,10 +104,16 @@ def tune_forest(forest, points, offsets, candidates=None, reps=3, use_cache=True
     if candidates is None:
         # second stage: the workgroup size of the best launch
         for wpb in WAVES_PER_BLOCK:
-            cfg = best[:3] + (wpb,)
+            cfg = best[:3] + (wpb, 0)
             if cfg not in times:
                 measure(cfg)
         best = min(times, key=times.get)
+        if best[1] == 0:  # third stage: the XCD chunk of a one-wave-per-tile grid
+            for xc in XCD_CHUNKS:
+                cfg = best[:4] + (xc,)
+                if cfg not in times:
+                    measure(cfg)
+            best = min(times, key=times.get)
     ms = {",".join(map(str, k)): v for k, v in times.items()}
     if use_cache:
         cache[key] = {"best": list(best), "ms": ms}

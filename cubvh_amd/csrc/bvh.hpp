@@ -9,7 +9,7 @@
 //   tris  : float4[3 * n_faces] in leaf order.  Triangle t occupies
 //           tris[3t+0] = (a.x, a.y, a.z, bits of the original face index),
 //           tris[3t+1] = (b.x, b.y, b.z, 0), tris[3t+2] = (c.x, c.y, c.z, 0).
-//           A leaf is a contiguous run of at most 8 triangles.
+//           A leaf is a contiguous run of at most MAX_LEAF_SIZE (16) triangles.
 //   tbnd  : float4[2 * n_faces] in leaf order, a bounding disk per face:
 //           tbnd[2t+0] = (centre, radius), tbnd[2t+1] = (unit normal, thickness).
 //           Every point of the face lies within `radius` of the centre in the plane

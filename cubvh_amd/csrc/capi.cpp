@@ -1105,8 +1105,8 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
                                    const int32_t* tuning, int32_t n_tuning, void* stream) {
     CUBVH_GUARD_BEGIN
     if (!forest) return fail(CUBVH_ERR_ARG, "forest is null");
-    if (n_tuning < 0 || n_tuning > 4 || (n_tuning > 0 && !tuning))
-        return fail(CUBVH_ERR_ARG, "tuning must hold 0 to 4 entries");
+    if (n_tuning < 0 || n_tuning > 5 || (n_tuning > 0 && !tuning))
+        return fail(CUBVH_ERR_ARG, "tuning must hold 0 to 5 entries");
     if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
     if (n < 0 || n >= ((int64_t)1 << 32) - 64 * (int64_t)forest->n_meshes)
         return fail(CUBVH_ERR_ARG, "query count out of range");
@@ -1116,7 +1116,9 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     const int32_t persistent = n_tuning > 1 ? tuning[1] : 1;
     const int32_t wpc = n_tuning > 2 ? tuning[2] : 8;
     const int32_t wpb = n_tuning > 3 ? tuning[3] : 4;
+    const int32_t xcd = n_tuning > 4 ? tuning[4] : 0;
     if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(CUBVH_ERR_ARG, "waves per block must be 1, 2, 4 or 8");
+    if (xcd < 0 || xcd > 1024) return fail(CUBVH_ERR_ARG, "XCD chunk must be in 0..1024 workgroups");
     if (bits < 0 || bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
     // a bin index is mesh << bits | code in 32 bits; the histogram holds one word per bin
     if (((int64_t)forest->n_meshes << bits) > kMaxForestBins)
@@ -1156,6 +1158,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     a.persistent = persistent;
     a.waves_per_cu = wpc;
     a.waves_per_block = wpb;
+    a.xcd_chunk = xcd;
     a.n_cu = c.n_cu;
     e = launch_forest_distance(a, ws, hs);
     if (e != hipSuccess) return fail_hip(e, "forest kernel launch");

@@ -43,6 +43,7 @@ struct ForestArgs {
     int32_t persistent = 0;
     int32_t waves_per_cu = 8;
     int32_t waves_per_block = 4;  // 1, 2, 4 or 8 (workgroups of 64..512 lanes)
+    int32_t xcd_chunk = 0;        // one-wave-per-tile grid: consecutive workgroups per XCD (0 = dispatch order)
     int32_t n_cu = 256;
 };
 

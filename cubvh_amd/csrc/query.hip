@@ -979,6 +979,20 @@ __global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const f
     if (uvw) store_pt(uvw, q, tri_barycentric(a, b, c, cp));
 }
 
+// Workgroup b of a launch runs on XCD b % 8: the dispatcher deals workgroups round-robin
+// over the 8 XCDs, each with its own L2.  xcd_remap gives workgroup b the logical block
+// it works on so that each XCD takes runs of `chunk` consecutive logical blocks (within
+// every group of 8 chunk workgroups; a last partial group keeps its order): neighbouring
+// tiles, which walk the same part of a tree, then share one L2 instead of eight.  A
+// bijection of [0, nb), so every tile is still walked once.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk) {
+    if (chunk == 0) return b;
+    const uint32_t group = 8u * chunk;
+    if (b >= nb / group * group) return b;
+    const uint32_t r = b % group;
+    return b - r + (r % 8u) * chunk + r / 8u;
+}
+
 // Unsigned distance over a forest: one wave per 64-slot tile of the padded query
 // order; every tile belongs to one mesh, whose root node and face-0 slot come from
 // mesh_info.  PERSISTENT waves take tiles from an atomic counter until none is left.
@@ -990,13 +1004,13 @@ __global__ __launch_bounds__(64 * WAVES) void forest_distance_kernel(const float
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
                                                               TreeArrays arrays, uint32_t* counter,
-                                                              int64_t max_tiles) {
+                                                              int64_t max_tiles, uint32_t xcd_chunk) {
     __shared__ ExactQueue queues[WAVES];
     const int lane = (int)(threadIdx.x & 63);
     // max_tiles: the tiles the order array holds (bounds the walk even when the offsets
     // are not a CSR of the queries)
     const int64_t n_tiles = min(pbase[n_meshes] / 64, max_tiles);
-    int64_t tile = PERSISTENT ? 0 : ((int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6));
+    int64_t tile = PERSISTENT ? 0 : ((int64_t)xcd_remap(blockIdx.x, gridDim.x, xcd_chunk) * WAVES + (threadIdx.x >> 6));
     for (;;) {
         if (PERSISTENT) {
             uint32_t t = 0;
@@ -1170,12 +1184,12 @@ hipError_t launch_forest_shape(const ForestArgs& a, const ForestWorkspace& ws, h
         const unsigned blocks = (unsigned)std::max(1, a.n_cu * a.waves_per_cu / WAVES);
         hipLaunchKernelGGL((dev::forest_distance_kernel<1, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, a.positions,
                            ws.order, ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree,
-                           ws.counter, max_tiles);
+                           ws.counter, max_tiles, 0u);
     } else {
         const unsigned blocks = (unsigned)((max_tiles + WAVES - 1) / WAVES);
         hipLaunchKernelGGL((dev::forest_distance_kernel<0, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, a.positions,
                            ws.order, ws.pbase, a.n_meshes, a.mesh_info, a.distances, a.face_id, a.uvw, a.tree,
-                           ws.counter, max_tiles);
+                           ws.counter, max_tiles, (uint32_t)a.xcd_chunk);
     }
     return hipGetLastError();
 }

@@ -19,8 +19,8 @@ import torch
 from . import _lib
 from .api import cuBVH, _query_tensor, _stream_of
 
-# Morton-prefix bits, persistent grid, waves per CU, waves per workgroup
-DEFAULT_TUNING = (12, 1, 8, 4)
+# Morton-prefix bits, persistent grid, waves per CU, waves per workgroup, XCD chunk
+DEFAULT_TUNING = (12, 1, 8, 4, 0)
 
 
 def _offsets_tensor(offsets, n, device, check=True):
@@ -99,8 +99,9 @@ class cuBVHForest:
         else:
             dist, face, uvw = out
         t = tuple(tuning) if tuning is not None else DEFAULT_TUNING
-        if not 1 <= len(t) <= 4:
-            raise ValueError("tuning holds 1 to 4 entries (bits, persistent, waves per CU, waves per workgroup)")
+        if not 1 <= len(t) <= 5:
+            raise ValueError("tuning holds 1 to 5 entries (bits, persistent, waves per CU, waves per workgroup, "
+                             "XCD chunk)")
         tarr = (ctypes.c_int32 * len(t))(*[int(x) for x in t])
         _lib.check(self._lib.cubvh_forest_unsigned_distance(
             self._h, dev.index, n, points.data_ptr(), offsets.data_ptr(), dist.data_ptr(), face.data_ptr(),

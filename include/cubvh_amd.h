@@ -229,11 +229,13 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
  * (offsets[0] = 0, non-decreasing, offsets[n_meshes] = n); they are read on the device
  * only, so other offsets are not reported: they give wrong answers, but every write
  * stays inside the outputs and the workspace.  Outputs as cubvh_unsigned_distance.
- * tuning: n_tuning (0..4) entries, the missing ones default to {12, 1, 8, 4}:
+ * tuning: n_tuning (0..5) entries, the missing ones default to {12, 1, 8, 4, 0}:
  * [0] Morton-prefix bits used to order the queries inside each mesh (0..16, 0 keeps
  * input order; n_meshes * 2^bits <= 2^30), [1] 1 = persistent grid taking 64-query
  * tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of the
- * persistent grid (1..32), [3] waves per workgroup (1, 2, 4 or 8).  Grows the workspace
+ * persistent grid (1..32), [3] waves per workgroup (1, 2, 4 or 8), [4] for the
+ * one-wave-per-tile grid, runs of that many consecutive workgroups given to each XCD
+ * (0..1024, 0 = dispatch order; neighbouring tiles then share one L2).  Grows the workspace
  * when the reserved one is too small, which is refused while the stream is being
  * captured. */
 int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,

@@ -287,8 +287,11 @@ def test_forest_morton_bins_limit_refused_before_device_work():
         assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, 4, None) == -1
         assert b"Morton bins" in lib.cubvh_last_error()
         # the tuning array's length is checked, and so is the workgroup shape
-        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, 5, None) == -1
-        assert b"0 to 4 entries" in lib.cubvh_last_error()
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, tun, 6, None) == -1
+        assert b"0 to 5 entries" in lib.cubvh_last_error()
+        bad_xcd = (ctypes.c_int32 * 5)(0, 0, 8, 4, -1)
+        assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, bad_xcd, 5, None) == -1
+        assert b"XCD chunk" in lib.cubvh_last_error()
         assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, None, 2, None) == -1
         bad = (ctypes.c_int32 * 4)(0, 1, 8, 3)
         assert lib.cubvh_forest_unsigned_distance(h, 0, 1000, 1, 1, 1, 1, None, bad, 4, None) == -1

@@ -37,7 +37,8 @@ def _per_mesh(bvhs, pts, offsets, dev):
 
 
 @pytest.mark.parametrize("tuning", [(12, 1, 8), (0, 0, 8), (9, 0, 8), (15, 1, 4), (6, 1, 16), (12, 1, 8, 1),
-                                    (12, 0, 8, 2), (9, 1, 4, 8), (0, 0, 8, 8), (15,)])
+                                    (12, 0, 8, 2), (9, 1, 4, 8), (0, 0, 8, 8), (15,), (15, 0, 8, 1, 16),
+                                    (9, 0, 8, 4, 4), (12, 0, 8, 2, 64), (12, 1, 8, 4, 16)])
 def test_forest_matches_per_mesh(gpu, tuning):
     bvhs, pts, offsets = _forest_case()
     forest = cubvh_amd.cuBVHForest(bvhs)
@@ -112,8 +113,11 @@ def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
     P = torch.from_numpy(pts).to(gpu)
     best, ms = cubvh_amd.tune_forest(forest, P, torch.from_numpy(offsets), reps=1)
     assert len(ms) >= 4 and all(v > 0 for v in ms.values())
-    # both stages ran: every workgroup size was tried with the best first-stage launch
-    assert len(best) == 4 and {int(k.split(",")[3]) for k in ms} == {1, 2, 4, 8}
+    # the stages ran: every workgroup size was tried with the best first-stage launch, and
+    # for a one-wave-per-tile winner every XCD chunk
+    assert len(best) == 5 and {int(k.split(",")[3]) for k in ms} == {1, 2, 4, 8}
+    if best[1] == 0:
+        assert {int(k.split(",")[4]) for k in ms} == {0, 4, 16, 64}
     best2, _ = cubvh_amd.tune_forest(forest, P, torch.from_numpy(offsets), reps=1)
     assert tuple(best2) == tuple(best)  # served from the cache
     d, f, _ = forest.unsigned_distance(P, torch.from_numpy(offsets), False, best)
