@@ -930,7 +930,7 @@ constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal se
 // Packet kernel for the unsigned (MODE 0) and watertight signed (MODE 1) distance.
 // Lane i of the grid handles query order[i] (or i without a permutation).
 template <int MODE>
-__global__ __launch_bounds__(256) void distance_packet_kernel(int64_t n, const float* __restrict__ positions,
+__global__ __launch_bounds__(256, 8) void distance_packet_kernel(int64_t n, const float* __restrict__ positions,
                                                               const uint32_t* __restrict__ order,
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
@@ -997,7 +997,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t 
 // order; every tile belongs to one mesh, whose root node and face-0 slot come from
 // mesh_info.  PERSISTENT waves take tiles from an atomic counter until none is left.
 template <int PERSISTENT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void forest_distance_kernel(const float* __restrict__ positions,
+__global__ __launch_bounds__(64 * WAVES, PERSISTENT ? 7 : 8) void forest_distance_kernel(const float* __restrict__ positions,
                                                               const uint32_t* __restrict__ order,
                                                               const int64_t* __restrict__ pbase, int32_t n_meshes,
                                                               const int4* __restrict__ mesh_info,
