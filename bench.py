@@ -73,16 +73,18 @@ def profile_figures(name):
 def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None, copy_gbs=None):
     """Roofline of one kernel: HBM bytes per launch from the counter summary (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy of known size) over `kernel_s`, the launch's average
-    duration measured here; the VALU figures of the same summary (busy = share of the
-    kernel's cycles the vector ALUs issue, lanes = active lanes per VALU instruction);
-    the algorithmic bytes (per-unit figure x units) over the same time when given."""
+    duration measured here; the VALU figures of the same summary (issue ratio = SIMD
+    cycles the VALU instructions need at 4 cycles per wave64 instruction over those
+    available, busy = rocprof's VALUBusy, lanes = active lanes per VALU instruction); the
+    algorithmic bytes (per-unit figure x units) over the same time when given."""
     f = profile_figures(name)
     traffic = f.get("traffic_bytes_per_launch")
     achieved = traffic / kernel_s / 1e9 if traffic and kernel_s > 0 else None
     blk = {"kernel": kernel, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
            "achieved": achieved, "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
            "traffic": traffic, "kernel_ms": kernel_s * 1e3,
-           "valu_busy": f.get("valu_busy"), "valu_lane_utilization": f.get("valu_lane_utilization"),
+           "valu_issue_ratio": f.get("valu_issue_ratio"), "valu_busy": f.get("valu_busy"),
+           "valu_lane_utilization": f.get("valu_lane_utilization"),
            "l2_hit_rate": f.get("l2_hit_rate"), "counters": f"profiles/{name}_counters.md" if f else None}
     if copy_gbs and achieved is not None:
         blk["copy_GBs"] = copy_gbs
@@ -92,11 +94,13 @@ def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None,
         blk["algorithmic_bytes_per_unit"] = algo_bytes_per_unit
         blk["algorithmic_GBs"] = algo
         blk["algorithmic_frac"] = algo / HBM_PEAK_GBS
-    busy = blk["valu_busy"]
-    if busy is not None and blk["frac"] is not None and busy > blk["frac"]:
-        blk["limiter"] = "valu issue (valu_busy above the HBM fraction)"
+    issue = blk["valu_issue_ratio"]
+    if issue is not None and issue >= 0.6:
+        blk["limiter"] = "VALU issue (SQ_INSTS_VALU x 4 cycles against the SIMD cycles)"
+    elif blk["frac"] is not None and blk["frac"] >= 0.5:
+        blk["limiter"] = "HBM"
     elif blk["frac"] is not None:
-        blk["limiter"] = "hbm"
+        blk["limiter"] = "latency (neither VALU issue nor HBM near its peak)"
     return blk
 
 
@@ -523,7 +527,7 @@ def run_extras(dev, steps, warmup, out, gpu_build=True):
     _, tu = _time_kernel(lambda: bvhs.unsigned_distance(Q), max(2, steps // 2), 1)
     # the stab rays' kernel: the raystab call minus the unsigned call it starts with
     out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3, "udf_part_ms": tu * 1e3,
-                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_init + stab_first + stab_rest + stab_negate", max(t - tu, 1e-9)),
+                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_init + stab_first + 4 x (stab_lines + stab_keep) + stab_negate", max(t - tu, 1e-9)),
                                                time_basis="raystab call minus the unsigned call")}
 
 

@@ -844,85 +844,122 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
 // and its distance negated, when all 64 stab rays from its point hit a face (the 32
 // stab lines, each hit both ways).  The stab directions are keyed by the query index,
 // as in distance_kernel.  Most queries are decided by their first line, a few need all
-// 32, so the lines are spread over the lanes in two launches instead of one lane
-// walking every line of its query:
+// 32.  A line walk is a chain of dependent node loads, so a launch lasts at least as
+// long as its longest walk; the lines therefore run in a few launches of many lines
+// each, every line of a batch in its own lane:
 //   stab_first : every query (in the Morton order) walks its line 0 and stores its two
-//                random offsets; the queries whose line hits both ways are appended to
-//                `list` (counts[0]) with their flag cleared;
-//   stab_rest  : one lane per (listed query, line 1..31), line-major, so that lines
-//                of a query already found outside (flag set) are skipped;
-//   stab_negate: the listed queries whose flag stayed clear are inside.
-// The decisions are the ones of walking a query's lines in order up to the first miss.
+//                random offsets; the queries whose line hits both ways are listed
+//                (list 0, counts[0]);
+//   stab_lines : lines k0 .. k0 + L - 1 of the listed queries, one lane per (line,
+//                entry); a line that does not hit both ways marks its entry (bit 31);
+//   stab_keep  : the unmarked entries make the next list (counts[b + 1]);
+//   stab_negate: the queries of the last list are inside.
+// Lines after a query's first miss may be walked too; the decisions are the ones of
+// walking a query's lines in order up to the first miss.  Lists alternate between two
+// buffers.  Queries number below 2^31, so bit 31 of an entry is free.
+constexpr int kStabLines = 32;
+constexpr uint32_t kStabMiss = 0x80000000u;
+// batches of lines after line 0: [1, 3), [3, 7), [7, 15), [15, 32), matched to the
+// roughly geometric decay of the undecided queries
+constexpr int kStabBatches = 4;
+constexpr uint32_t kStabBatchFirst[kStabBatches + 1] = {1u, 3u, 7u, 15u, 32u};
+
 __global__ void stab_init(uint32_t* counts) {
-    if (threadIdx.x == 0) counts[0] = 0u;
+    if (threadIdx.x < (unsigned)kStabLines) counts[threadIdx.x] = 0u;
+}
+
+// Appends q of every lane with keep set to list (count words used), one atomic per
+// wave; every lane of the wave must reach it.
+__device__ __forceinline__ void append_kept(bool keep, uint32_t q, uint32_t* __restrict__ list, uint32_t* count) {
+    const uint64_t m = ballot(keep);
+    if (m == 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = (int)__builtin_ctzll(m);
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
+    b = (uint32_t)__builtin_amdgcn_readlane((int32_t)b, leader);
+    if (keep) list[b + lanes_below(m)] = q;
 }
 
 template <int BLOCK, int LDS_N>
 __global__ __launch_bounds__(BLOCK) void stab_first(int64_t n, const uint32_t* __restrict__ order,
                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
-                                                    uint32_t* __restrict__ flags, const float* __restrict__ positions,
-                                                    float2* __restrict__ offs, Tree T) {
+                                                    const float* __restrict__ positions, float2* __restrict__ offs,
+                                                    Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
     st.tid = (int)threadIdx.x;
     st.sp = 0;
-    const int lane = (int)(threadIdx.x & 63);
-    const uint32_t q = order[i];
-    Pcg32 rng;
-    rng.advance((uint64_t)(uint32_t)(q * 2u));
-    float2 o;
-    o.x = rng.next_float();
-    o.y = rng.next_float();
-    offs[q] = o;
-    const bool keep = any_hit_line(T, load_pt(positions, q), fibonacci_dir32(0u, o.x, o.y), st) == 3;
-    const uint64_t m = ballot(keep);
-    if (m == 0) return;
-    const int leader = (int)__builtin_ctzll(m);
-    uint32_t b = 0;
-    if (lane == leader) b = atomicAdd(&counts[0], (uint32_t)__builtin_popcountll(m));
-    b = (uint32_t)__builtin_amdgcn_readlane((int32_t)b, leader);
-    if (keep) {
-        list[b + lanes_below(m)] = q;
-        flags[q] = 0u;
+    bool keep = false;
+    uint32_t q = 0;
+    if (i < n) {
+        q = order[i];
+        Pcg32 rng;
+        rng.advance((uint64_t)(uint32_t)(q * 2u));
+        float2 o;
+        o.x = rng.next_float();
+        o.y = rng.next_float();
+        offs[q] = o;
+        keep = any_hit_line(T, load_pt(positions, q), fibonacci_dir32(0u, o.x, o.y), st) == 3;
     }
+    append_kept(keep, q, list, &counts[0]);
 }
 
+// Grid: n_lines x (blocks for the longest possible list); the list length m is read on
+// the device, block b walks line k0 + b / ceil(m / BLOCK) and the blocks past the last
+// line leave.  One lane per (line, entry), no loop: a loop over several items per lane
+// keeps the walk's registers live across iterations (87 VGPRs instead of 56).
 template <int BLOCK, int LDS_N>
-__global__ __launch_bounds__(BLOCK) void stab_rest(const uint32_t* __restrict__ list,
-                                                   const uint32_t* __restrict__ counts, uint32_t* __restrict__ flags,
-                                                   const float* __restrict__ positions,
-                                                   const float2* __restrict__ offs, Tree T) {
+__global__ __launch_bounds__(BLOCK) void stab_lines(uint32_t k0, uint32_t n_lines, uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count, const float* __restrict__ positions,
+                                                    const float2* __restrict__ offs, Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
+    const uint32_t m = *count;
+    if (m == 0) return;
+    const uint32_t nb = (m + (uint32_t)BLOCK - 1u) / (uint32_t)BLOCK;
+    const uint32_t line = blockIdx.x / nb;
+    if (line >= n_lines) return;
+    const uint32_t j = (blockIdx.x - line * nb) * (uint32_t)BLOCK + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t e = __hip_atomic_load(&list[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (e & kStabMiss) return;  // an earlier line of the batch already missed
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
     st.tid = (int)threadIdx.x;
     st.sp = 0;
-    const int64_t m = (int64_t)counts[0];
-    const int64_t work = m * 31;  // item t: line 1 + t / m of query list[t % m]
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < work; t += stride) {
-        const uint32_t k = 1u + (uint32_t)(t / m);
-        const uint32_t q = list[t % m];
-        if (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) continue;
-        const float2 o = offs[q];
-        if (any_hit_line(T, load_pt(positions, q), fibonacci_dir32(k, o.x, o.y), st) != 3)
-            atomicOr(&flags[q], 1u);
-    }
+    const float2 o = offs[e];
+    if (any_hit_line(T, load_pt(positions, e), fibonacci_dir32(k0 + line, o.x, o.y), st) != 3)
+        __hip_atomic_fetch_or(&list[j], kStabMiss, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(256) void stab_negate(const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts,
-                                                   const uint32_t* __restrict__ flags, float* __restrict__ distances) {
+// The unmarked entries of list_in (count_in of them) make list_out (count_out).
+__global__ __launch_bounds__(256) void stab_keep(const uint32_t* __restrict__ list_in, const uint32_t* __restrict__ count_in,
+                                                 uint32_t* __restrict__ list_out, uint32_t* __restrict__ count_out) {
+    const uint32_t m = *count_in;
+    const uint32_t wave0 = blockIdx.x * 256u + (threadIdx.x & ~63u);
+    if (wave0 >= m) return;  // whole waves past the end; the others all reach append_kept
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    bool keep = false;
+    uint32_t q = 0;
+    if (j < m) {
+        q = list_in[j];
+        keep = (q & kStabMiss) == 0u;
+    }
+    append_kept(keep, q, list_out, count_out);
+}
+
+__global__ __launch_bounds__(256) void stab_negate(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                                   float* __restrict__ distances) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)counts[0]) return;
+    if (i >= (int64_t)*count) return;
     const uint32_t q = list[i];
-    if (flags[q] == 0u) distances[q] = -distances[q];
+    distances[q] = -distances[q];
 }
 
 constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal search
@@ -1223,32 +1260,33 @@ size_t stab_workspace_bytes(int64_t n) {
 hipError_t launch_stab_sign(const QueryArgs& q, void* ws) {
     if (q.n <= 0) return hipSuccess;
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
-    if (!q.order || !ws || q.n >= ((int64_t)1 << 32)) return hipErrorInvalidValue;
+    if (!q.order || !ws || q.n >= ((int64_t)1 << 31)) return hipErrorInvalidValue;  // bit 31 marks list entries
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     hipStream_t s = (hipStream_t)q.stream;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     char* w = (char*)ws;
     uint32_t* counts = (uint32_t*)w;
     w += al(64);
-    uint32_t* list = (uint32_t*)w;
+    uint32_t* lists[2];
+    lists[0] = (uint32_t*)w;
     w += al(4 * (size_t)q.n);
-    uint32_t* flags = (uint32_t*)w;
+    lists[1] = (uint32_t*)w;
     w += al(4 * (size_t)q.n);
     float2* offs = (float2*)w;
-    int dev_id = 0, n_cu = 256;
-    (void)hipGetDevice(&dev_id);
-    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev_id);
     const int64_t full = (q.n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts);
     hipLaunchKernelGGL((dev::stab_first<kBlock, kLdsEntries>), dim3((unsigned)full), dim3(kBlock), 0, s, q.n, q.order,
-                       list, counts, flags, q.in0, offs, T);
-    // lines 1..31 of the listed queries: a grid that fills the device, striding over
-    // (list length x 31) items read on the device
-    const unsigned rest = (unsigned)std::min<int64_t>(full * 31, (int64_t)n_cu * 32);
-    hipLaunchKernelGGL((dev::stab_rest<kBlock, kLdsEntries>), dim3(rest), dim3(kBlock), 0, s, list, counts, flags, q.in0,
-                       offs, T);
-    hipLaunchKernelGGL(dev::stab_negate, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s, list, counts, flags,
-                       q.out_f);
+                       lists[0], counts, q.in0, offs, T);
+    // list lengths are known on the device only: each launch covers the longest possible
+    for (int b = 0; b < dev::kStabBatches; ++b) {
+        const uint32_t k0 = dev::kStabBatchFirst[b], nl = dev::kStabBatchFirst[b + 1] - k0;
+        hipLaunchKernelGGL((dev::stab_lines<kBlock, kLdsEntries>), dim3((unsigned)(full * nl)), dim3(kBlock), 0, s, k0,
+                           nl, lists[b & 1], counts + b, q.in0, offs, T);
+        hipLaunchKernelGGL(dev::stab_keep, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s, lists[b & 1],
+                           counts + b, lists[(b + 1) & 1], counts + b + 1);
+    }
+    hipLaunchKernelGGL(dev::stab_negate, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s,
+                       lists[dev::kStabBatches & 1], counts + dev::kStabBatches, q.out_f);
     return hipGetLastError();
 }
 
