@@ -301,16 +301,30 @@ __device__ int32_t closest_hit(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N
 // its key and is dropped at pop when both are answered.  open1 / open2 count the stack
 // entries still open for +d / -d: a direction not yet hit with no open entry and not
 // open in the subtree at hand can no longer be hit.
-template <int BLOCK, int LDS_N>
-__device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
-    const f3 inv = ray_inverse(d);
-    const float limit = MAX_DIST * BOUND_SLACK;
-    int found = 0;
-    int32_t ref = 0;
-    int dirs = 3;
-    int open1 = 0, open2 = 0;
-    st.sp = 0;
-    for (;;) {
+// The walk as a resumable state: init, then step until it returns a result (>= 0), so
+// that a lane can start the next line as soon as its own ends (persistent stab kernels).
+struct LineWalk {
+    f3 o, d, inv;
+    int found, dirs, open1, open2;
+    int32_t ref;
+
+    template <int BLOCK, int LDS_N>
+    __device__ __forceinline__ void init(f3 o_, f3 d_, LaneStack<BLOCK, LDS_N>& st) {
+        o = o_;
+        d = d_;
+        inv = ray_inverse(d_);
+        found = 0;
+        ref = 0;
+        dirs = 3;
+        open1 = open2 = 0;
+        st.sp = 0;
+    }
+
+    // One node or leaf visit (and the pop after it): -1 while the pair is undecided,
+    // else the walk's result as any_hit_line returns it.
+    template <int BLOCK, int LDS_N>
+    __device__ __forceinline__ int step(const Tree& T, LaneStack<BLOCK, LDS_N>& st) {
+        const float limit = MAX_DIST * BOUND_SLACK;
         if ((dirs & ~found) != 0) {
             if (ref >= 0) {
                 WideNode nd = load_node(T.nodes, ref);
@@ -346,7 +360,7 @@ __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& 
                     dirs = (int)(__float_as_uint(k[0]) & 3u);
                     const int pending = dirs | (open1 ? 1 : 0) | (open2 ? 2 : 0);
                     if ((3 & ~found & ~pending) != 0) return found;  // a direction without a hit left
-                    continue;
+                    return -1;
                 }
             } else {
                 int32_t first, count;
@@ -367,6 +381,17 @@ __device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& 
         dirs = (int)(__float_as_uint(st.key_at(st.sp)) & 3u);
         open1 -= dirs & 1;
         open2 -= dirs >> 1;
+        return -1;
+    }
+};
+
+template <int BLOCK, int LDS_N>
+__device__ int any_hit_line(const Tree& T, f3 o, f3 d, LaneStack<BLOCK, LDS_N>& st) {
+    LineWalk w;
+    w.init(o, d, st);
+    for (;;) {
+        const int r = w.step(T, st);
+        if (r >= 0) return r;
     }
 }
 
@@ -844,28 +869,35 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
 // and its distance negated, when all 64 stab rays from its point hit a face (the 32
 // stab lines, each hit both ways).  The stab directions are keyed by the query index,
 // as in distance_kernel.  Most queries are decided by their first line, a few need all
-// 32.  A line walk is a chain of dependent node loads, so a launch lasts at least as
-// long as its longest walk; the lines therefore run in a few launches of many lines
-// each, every line of a batch in its own lane:
-//   stab_first : every query (in the Morton order) walks its line 0 and stores its two
-//                random offsets; the queries whose line hits both ways are listed
-//                (list 0, counts[0]);
-//   stab_lines : lines k0 .. k0 + L - 1 of the listed queries, one lane per (line,
-//                entry); a line that does not hit both ways marks its entry (bit 31);
-//   stab_keep  : the unmarked entries make the next list (counts[b + 1]);
+// 32.  A line walk is a chain of dependent node loads of very uneven length: a launch
+// lasts at least as long as its longest walk, so the lines after the first run in a
+// few batches of many lines each, in persistent waves whose idle lanes take the next
+// line from a counter (a wave no longer waits for its longest walk):
+//   stab_first : line 0 of every query (in the Morton order), one lane per query, which
+//                also stores the query's two random offsets; the queries whose line
+//                hits both ways make list 0;
+//   stab_lines : lines k0 .. k0 + L - 1 of the listed queries, one item per (line,
+//                entry), line-major; a line that misses marks its entry (bit 31);
+//   stab_keep  : the unmarked entries make the next list;
 //   stab_negate: the queries of the last list are inside.
 // Lines after a query's first miss may be walked too; the decisions are the ones of
-// walking a query's lines in order up to the first miss.  Lists alternate between two
-// buffers.  Queries number below 2^31, so bit 31 of an entry is free.
-constexpr int kStabLines = 32;
+// walking a query's lines in order up to the first miss.  Queries number below 2^31,
+// so bit 31 of an entry is free.  counts[0] = n, counts[1 + b] = length of list b;
+// item counters are 64-bit words from counts + 32.
 constexpr uint32_t kStabMiss = 0x80000000u;
 // batches of lines after line 0: [1, 3), [3, 7), [7, 15), [15, 32), matched to the
 // roughly geometric decay of the undecided queries
 constexpr int kStabBatches = 4;
 constexpr uint32_t kStabBatchFirst[kStabBatches + 1] = {1u, 3u, 7u, 15u, 32u};
+// idle lanes of a wave take new items once this many of the 64 are idle (or all are)
+constexpr int kRefillLanes = 16;
+// waves per SIMD the persistent stab kernels are compiled for (80 VGPRs); with 12 KB
+// of LDS per 2-wave workgroup, 12 workgroups per CU
+constexpr int kStabWaves = 6;
+constexpr int kStabBlocksPerCu = 2 * kStabWaves;
 
-__global__ void stab_init(uint32_t* counts) {
-    if (threadIdx.x < (unsigned)kStabLines) counts[threadIdx.x] = 0u;
+__global__ void stab_init(uint32_t* counts, int64_t n) {
+    counts[threadIdx.x] = threadIdx.x == 0 ? (uint32_t)n : 0u;  // 64 threads: the 256-byte header
 }
 
 // Appends q of every lane with keep set to list (count words used), one atomic per
@@ -881,11 +913,79 @@ __device__ __forceinline__ void append_kept(bool keep, uint32_t q, uint32_t* __r
     if (keep) list[b + lanes_below(m)] = q;
 }
 
+// Persistent line walks: every lane walks items job.start(t) for t taken from
+// *counter until the counter passes job.total; job.finish gets each result.  Every
+// lane of the wave returns at the same time.
+template <int BLOCK, int LDS_N, class Job>
+__device__ void walk_lines(const Tree& T, const Job& job, unsigned long long* counter, LaneStack<BLOCK, LDS_N>& st) {
+    const int lane = (int)(threadIdx.x & 63);
+    LineWalk w;
+    bool live = false, more = true;
+    uint32_t tag = 0;
+    for (;;) {
+        const uint64_t idle = ballot(!live && more);
+        if (idle != 0) {
+            const uint64_t busy = ballot(live);
+            if (busy == 0 || __builtin_popcountll(idle) >= kRefillLanes) {
+                const int leader = (int)__builtin_ctzll(idle);
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(counter, (unsigned long long)__builtin_popcountll(idle));
+                base = (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)base, leader) |
+                       ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(base >> 32), leader) << 32);
+                if (!live && more) {
+                    const unsigned long long t = base + (unsigned long long)lanes_below(idle);
+                    if (t >= job.total) {
+                        more = false;
+                    } else {
+                        f3 o, d;
+                        if (job.start(t, o, d, tag)) {
+                            w.init(o, d, st);
+                            live = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (ballot(live) == 0) {
+            if (ballot(more) == 0) return;
+            continue;
+        }
+        if (live) {
+            const int r = w.step(T, st);
+            if (r >= 0) {
+                job.finish(tag, r);
+                live = false;
+            }
+        }
+    }
+}
+
+struct LineBatchJob {  // item t: line k0 + t / m of entry t % m
+    unsigned long long total;
+    uint32_t m, k0;
+    uint32_t* list;
+    const float* positions;
+    const float2* offs;
+    __device__ __forceinline__ bool start(unsigned long long t, f3& o, f3& d, uint32_t& tag) const {
+        const uint32_t line = (uint32_t)(t / m);
+        const uint32_t j = (uint32_t)(t - (unsigned long long)line * m);
+        const uint32_t e = __hip_atomic_load(&list[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e & kStabMiss) return false;  // an earlier line of the batch already missed
+        const float2 r = offs[e];
+        o = load_pt(positions, e);
+        d = fibonacci_dir32(k0 + line, r.x, r.y);
+        tag = j;
+        return true;
+    }
+    __device__ __forceinline__ void finish(uint32_t tag, int found) const {
+        if (found != 3) __hip_atomic_fetch_or(&list[tag], kStabMiss, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
 template <int BLOCK, int LDS_N>
-__global__ __launch_bounds__(BLOCK) void stab_first(int64_t n, const uint32_t* __restrict__ order,
-                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
-                                                    const float* __restrict__ positions, float2* __restrict__ offs,
-                                                    Tree T) {
+__global__ __launch_bounds__(BLOCK) void stab_first(const uint32_t* __restrict__ order, uint32_t* __restrict__ list,
+                                                    uint32_t* __restrict__ counts, const float* __restrict__ positions,
+                                                    float2* __restrict__ offs, Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -896,7 +996,7 @@ __global__ __launch_bounds__(BLOCK) void stab_first(int64_t n, const uint32_t* _
     st.sp = 0;
     bool keep = false;
     uint32_t q = 0;
-    if (i < n) {
+    if (i < (int64_t)counts[0]) {
         q = order[i];
         Pcg32 rng;
         rng.advance((uint64_t)(uint32_t)(q * 2u));
@@ -906,36 +1006,28 @@ __global__ __launch_bounds__(BLOCK) void stab_first(int64_t n, const uint32_t* _
         offs[q] = o;
         keep = any_hit_line(T, load_pt(positions, q), fibonacci_dir32(0u, o.x, o.y), st) == 3;
     }
-    append_kept(keep, q, list, &counts[0]);
+    append_kept(keep, q, list, &counts[1]);
 }
 
-// Grid: n_lines x (blocks for the longest possible list); the list length m is read on
-// the device, block b walks line k0 + b / ceil(m / BLOCK) and the blocks past the last
-// line leave.  One lane per (line, entry), no loop: a loop over several items per lane
-// keeps the walk's registers live across iterations (87 VGPRs instead of 56).
 template <int BLOCK, int LDS_N>
-__global__ __launch_bounds__(BLOCK) void stab_lines(uint32_t k0, uint32_t n_lines, uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count, const float* __restrict__ positions,
+__global__ __launch_bounds__(BLOCK, kStabWaves) void stab_lines(uint32_t b, uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
+                                                    const float* __restrict__ positions,
                                                     const float2* __restrict__ offs, Tree T) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
-    const uint32_t m = *count;
-    if (m == 0) return;
-    const uint32_t nb = (m + (uint32_t)BLOCK - 1u) / (uint32_t)BLOCK;
-    const uint32_t line = blockIdx.x / nb;
-    if (line >= n_lines) return;
-    const uint32_t j = (blockIdx.x - line * nb) * (uint32_t)BLOCK + threadIdx.x;
-    if (j >= m) return;
-    const uint32_t e = __hip_atomic_load(&list[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (e & kStabMiss) return;  // an earlier line of the batch already missed
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
     st.tid = (int)threadIdx.x;
     st.sp = 0;
-    const float2 o = offs[e];
-    if (any_hit_line(T, load_pt(positions, e), fibonacci_dir32(k0 + line, o.x, o.y), st) != 3)
-        __hip_atomic_fetch_or(&list[j], kStabMiss, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    LineBatchJob job;
+    job.m = counts[1 + b];
+    job.k0 = kStabBatchFirst[b];
+    job.total = (unsigned long long)job.m * (kStabBatchFirst[b + 1] - kStabBatchFirst[b]);
+    job.list = list;
+    job.positions = positions;
+    job.offs = offs;
+    walk_lines(T, job, (unsigned long long*)(counts + 32) + 1 + b, st);
 }
 
 // The unmarked entries of list_in (count_in of them) make list_out (count_out).
@@ -1273,20 +1365,23 @@ hipError_t launch_stab_sign(const QueryArgs& q, void* ws) {
     lists[1] = (uint32_t*)w;
     w += al(4 * (size_t)q.n);
     float2* offs = (float2*)w;
-    const int64_t full = (q.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts);
-    hipLaunchKernelGGL((dev::stab_first<kBlock, kLdsEntries>), dim3((unsigned)full), dim3(kBlock), 0, s, q.n, q.order,
-                       lists[0], counts, q.in0, offs, T);
-    // list lengths are known on the device only: each launch covers the longest possible
+    int dev_id = 0, n_cu = 256;
+    (void)hipGetDevice(&dev_id);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev_id);
+    // persistent waves: as many workgroups as the CUs hold at once (12 KB of LDS each)
+    const unsigned grid = (unsigned)std::min<int64_t>((q.n + kBlock - 1) / kBlock, (int64_t)n_cu * dev::kStabBlocksPerCu);
+    const unsigned keep_grid = (unsigned)((q.n + 255) / 256);
+    hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts, q.n);
+    hipLaunchKernelGGL((dev::stab_first<kBlock, kLdsEntries>), dim3((unsigned)((q.n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, q.order, lists[0], counts, q.in0, offs, T);
     for (int b = 0; b < dev::kStabBatches; ++b) {
-        const uint32_t k0 = dev::kStabBatchFirst[b], nl = dev::kStabBatchFirst[b + 1] - k0;
-        hipLaunchKernelGGL((dev::stab_lines<kBlock, kLdsEntries>), dim3((unsigned)(full * nl)), dim3(kBlock), 0, s, k0,
-                           nl, lists[b & 1], counts + b, q.in0, offs, T);
-        hipLaunchKernelGGL(dev::stab_keep, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s, lists[b & 1],
-                           counts + b, lists[(b + 1) & 1], counts + b + 1);
+        hipLaunchKernelGGL((dev::stab_lines<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, s, (uint32_t)b, lists[b & 1],
+                           counts, q.in0, offs, T);
+        hipLaunchKernelGGL(dev::stab_keep, dim3(keep_grid), dim3(256), 0, s, lists[b & 1], counts + 1 + b,
+                           lists[(b + 1) & 1], counts + 2 + b);
     }
-    hipLaunchKernelGGL(dev::stab_negate, dim3((unsigned)((q.n + 255) / 256)), dim3(256), 0, s,
-                       lists[dev::kStabBatches & 1], counts + dev::kStabBatches, q.out_f);
+    hipLaunchKernelGGL(dev::stab_negate, dim3(keep_grid), dim3(256), 0, s, lists[dev::kStabBatches & 1],
+                       counts + 1 + dev::kStabBatches, q.out_f);
     return hipGetLastError();
 }
 
