@@ -70,22 +70,57 @@ def profile_figures(name):
     return out
 
 
+# the device's VALU issue ceiling in wave64 instructions per second, measured once per
+# run by valu_ceiling() (cubvh_valu_probe)
+_VALU_PEAK = {}
+
+
+def valu_ceiling(dev, iters=20000, reps=3):
+    """Best of `reps` runs of each form of cubvh_valu_probe (8 waves per SIMD of
+    independent fp32 fma chains; form 0 with VGPR and literal operands, form 1 with an
+    SGPR operand), in wave64 instructions per second."""
+    import ctypes
+
+    import cubvh_amd
+
+    lib = cubvh_amd._lib.load()
+    for form, key in ((0, "wave_instr_per_s"), (1, "sgpr_wave_instr_per_s")):
+        best = 0.0
+        for _ in range(reps):
+            wi = ctypes.c_double(0.0)
+            ms = ctypes.c_float(0.0)
+            cubvh_amd._lib.check(lib.cubvh_valu_probe(dev.index, form, iters, ctypes.byref(wi), ctypes.byref(ms)))
+            if ms.value > 0:
+                best = max(best, wi.value / (ms.value * 1e-3))
+        _VALU_PEAK[key] = best
+    return dict(_VALU_PEAK)
+
+
 def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None, copy_gbs=None):
     """Roofline of one kernel: HBM bytes per launch from the counter summary (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy of known size) over `kernel_s`, the launch's average
-    duration measured here; the VALU figures of the same summary (issue ratio = SIMD
-    cycles the VALU instructions need at 4 cycles per wave64 instruction over those
-    available, busy = rocprof's VALUBusy, lanes = active lanes per VALU instruction); the
-    algorithmic bytes (per-unit figure x units) over the same time when given."""
+    duration measured here; its VALU issue rate (SQ_INSTS_VALU per launch of the same
+    summary over `kernel_s`) against the ceiling valu_ceiling() measured in this run;
+    active lanes per VALU instruction and the L2 hit rate of the summary; the algorithmic
+    bytes (per-unit figure x units) over the same time when given."""
     f = profile_figures(name)
     traffic = f.get("traffic_bytes_per_launch")
     achieved = traffic / kernel_s / 1e9 if traffic and kernel_s > 0 else None
     blk = {"kernel": kernel, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
            "achieved": achieved, "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
            "traffic": traffic, "kernel_ms": kernel_s * 1e3,
-           "valu_issue_ratio": f.get("valu_issue_ratio"), "valu_busy": f.get("valu_busy"),
            "valu_lane_utilization": f.get("valu_lane_utilization"),
            "l2_hit_rate": f.get("l2_hit_rate"), "counters": f"profiles/{name}_counters.md" if f else None}
+    insts = f.get("sq_insts_valu")
+    peak = _VALU_PEAK.get("wave_instr_per_s")
+    if insts and kernel_s > 0:
+        rate = insts / kernel_s
+        sg = _VALU_PEAK.get("sgpr_wave_instr_per_s")
+        blk["valu"] = {"insts_per_launch": insts, "achieved_wave_instr_per_s": rate,
+                       "peak_wave_instr_per_s": peak, "frac": rate / peak if peak else None,
+                       "sgpr_form_wave_instr_per_s": sg, "frac_of_sgpr_form": rate / sg if sg else None,
+                       "peak_source": "cubvh_valu_probe: 8 waves per SIMD, 16 independent fp32 fma chains; "
+                                      "peak = VGPR and literal operands, sgpr_form = one SGPR operand (half rate)"}
     if copy_gbs and achieved is not None:
         blk["copy_GBs"] = copy_gbs
         blk["frac_of_copy"] = achieved / copy_gbs
@@ -94,9 +129,9 @@ def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None,
         blk["algorithmic_bytes_per_unit"] = algo_bytes_per_unit
         blk["algorithmic_GBs"] = algo
         blk["algorithmic_frac"] = algo / HBM_PEAK_GBS
-    issue = blk["valu_issue_ratio"]
-    if issue is not None and issue >= 0.6:
-        blk["limiter"] = "VALU issue (SQ_INSTS_VALU x 4 cycles against the SIMD cycles)"
+    vf = blk.get("valu", {}).get("frac")
+    if vf is not None and vf >= 0.6:
+        blk["limiter"] = "VALU issue (SQ_INSTS_VALU per second against the measured fp32 fma issue ceiling)"
     elif blk["frac"] is not None and blk["frac"] >= 0.5:
         blk["limiter"] = "HBM"
     elif blk["frac"] is not None:
@@ -549,6 +584,8 @@ def main():
                     help="run only the sparse-voxel hash table leg and print its JSON")
     ap.add_argument("--smc-only", action="store_true",
                     help="run only the sparse marching cubes leg and print its JSON")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="run only the VALU issue-ceiling probe (cubvh_valu_probe) and print its rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the query path runs on one "
@@ -562,6 +599,9 @@ def main():
     from cubvh_amd import synthetic as S
 
     dev = torch.device("cuda:0")
+    if args.valu_probe:
+        print(json.dumps({"valu_ceiling": valu_ceiling(dev)}), flush=True)
+        return
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -623,6 +663,7 @@ def main():
     ev = float(np.mean([a.elapsed_time(b) for a, b in zip(k0, k1)])) / 1e3
     value = n * args.steps / wall / 1e6
     dropin_wall, _ = _time_kernel(dropin, args.steps, 1)
+    valu_ceiling(dev)
     ab = _algo_bytes()
     roof = None
     if args.res == 256:

@@ -35,6 +35,7 @@ SIGNATURES = {
         c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     ),
     "cubvh_morton_order": (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p]),
+    "cubvh_valu_probe": (c_int, [c_int, c_int32, c_int32, c_void_p, c_void_p]),
     "cubvh_distance_ordered": (
         c_int,
         [c_void_p, c_int, c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

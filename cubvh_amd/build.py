@@ -18,7 +18,7 @@ REPO_DIR = PKG_DIR.parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libcubvh_amd.so"
 
-SOURCES = ["capi.cpp", "build.cpp", "query.hip", "order.hip", "forest.hip", "floodfill.hip", "grad.hip", "gpubuild.hip", "hashtable.hip", "smc.hip"]
+SOURCES = ["capi.cpp", "build.cpp", "query.hip", "order.hip", "forest.hip", "floodfill.hip", "grad.hip", "gpubuild.hip", "hashtable.hip", "smc.hip", "probe.hip"]
 
 ARCH = os.environ.get("CUBVH_AMD_ARCH", "gfx950")
 

@@ -785,6 +785,41 @@ int cubvh_unsigned_distance_noalloc(cubvh_bvh* bvh, int device, int64_t n, const
     CUBVH_GUARD_END
 }
 
+int cubvh_valu_probe(int device, int32_t form, int32_t iters, double* wave_instructions, float* ms) {
+    CUBVH_GUARD_BEGIN
+    if (!wave_instructions || !ms) return fail(CUBVH_ERR_ARG, "null output");
+    if (form != 0 && form != 1) return fail(CUBVH_ERR_ARG, "form must be 0 (VGPR operands) or 1 (an SGPR operand)");
+    if (iters < 1 || iters > (1 << 20)) return fail(CUBVH_ERR_ARG, "iters must be in [1, 2^20]");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    int n_cu = 0;
+    hipError_t e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return fail_hip(e, "hipDeviceGetAttribute");
+    const int blocks = n_cu * 8;  // 4 waves per workgroup: 8 waves on each of a CU's 4 SIMDs
+    float* out = nullptr;
+    e = hipMalloc(&out, (size_t)blocks * 256 * sizeof(float));
+    if (e != hipSuccess) return fail_hip(e, "hipMalloc(probe output)");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_valu_probe(form, blocks, iters, out, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    hipError_t e2 = hipFree(out);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail_hip(e, "VALU probe");
+    *wave_instructions = (double)blocks * 4.0 * (double)iters * (double)kValuProbeChains;
+    *ms = t;
+    return CUBVH_OK;
+    CUBVH_GUARD_END
+}
+
 int cubvh_morton_order(int device, int64_t n, const float* positions, uint32_t* order, void* stream) {
     CUBVH_GUARD_BEGIN
     if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");

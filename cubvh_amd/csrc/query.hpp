@@ -45,6 +45,12 @@ hipError_t launch_ray(const QueryArgs& q);
 size_t stab_workspace_bytes(int64_t n);
 hipError_t launch_stab_sign(const QueryArgs& q, void* ws);
 
+// The VALU issue-ceiling probe (probe.hip): `blocks` workgroups of 256 threads, each
+// thread kValuProbeChains independent fp32 fma chains of `iters` steps; form 0 = VGPR
+// and literal operands, 1 = one SGPR operand; out holds blocks x 256 floats.
+constexpr int kValuProbeChains = 16;
+hipError_t launch_valu_probe(int form, int blocks, int32_t iters, float* out, hipStream_t s);
+
 // Morton permutation of point queries (order.hip).
 size_t morton_workspace_bytes(int64_t n);
 hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* order_out, hipStream_t s);

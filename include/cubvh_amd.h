@@ -321,6 +321,18 @@ int cubvh_smc_write(cubvh_smc* smc, float* vertices, int32_t* triangles);
 /* Releases the scratch (stream-ordered, after the writes) and the handle. */
 void cubvh_smc_destroy(cubvh_smc* smc);
 
+/* ---------------------------------------------------------------------------
+ * Measurement aid (no counterpart in the reference): the device's VALU issue
+ * ceiling.  Runs 8 waves on every SIMD, each lane 16 independent fp32 fma chains of
+ * `iters` steps (1 <= iters <= 2^20), on the null stream, and synchronizes.  form 0:
+ * operands in VGPRs and literals (the fastest fp32 form); form 1: one operand in an
+ * SGPR (the form of arithmetic on wave-uniform node and face records), which issues
+ * at about half the rate on gfx950.  *wave_instructions = the fma instructions issued
+ * (counted per wave64), *ms = the kernel time.  The walk kernels are bounded by VALU
+ * issue; their SQ_INSTS_VALU per second over these rates places them between the two.
+ * ------------------------------------------------------------------------- */
+int cubvh_valu_probe(int device, int32_t form, int32_t iters, double* wave_instructions, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -416,3 +416,23 @@ def test_export_reference_layout_walked_by_reference_traversal(oracle):
         # and it imports back (the reference layout importer rebuilds this layout)
         b2 = cubvh_amd.cuBVH.from_state_dict(sd)
         np.testing.assert_array_equal(b2._vertices.reshape(-1, 3, 3), v[f])
+
+
+def test_valu_probe_argument_checks():
+    """cubvh_valu_probe refuses null outputs, forms other than 0 and 1, iteration counts
+    outside [1, 2^20] and bad device ordinals before it touches a device."""
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    wi = ctypes.c_double(0.0)
+    ms = ctypes.c_float(0.0)
+    assert lib.cubvh_valu_probe(0, 0, 100, None, ctypes.byref(ms)) == -1
+    assert b"null output" in lib.cubvh_last_error()
+    for form in (-1, 2):
+        assert lib.cubvh_valu_probe(0, form, 100, ctypes.byref(wi), ctypes.byref(ms)) == -1
+        assert b"form" in lib.cubvh_last_error()
+    for iters in (0, -5, (1 << 20) + 1):
+        assert lib.cubvh_valu_probe(0, 0, iters, ctypes.byref(wi), ctypes.byref(ms)) == -1
+        assert b"iters" in lib.cubvh_last_error()
+    assert lib.cubvh_valu_probe(-1, 0, 100, ctypes.byref(wi), ctypes.byref(ms)) == -1
+    assert b"device ordinal" in lib.cubvh_last_error()

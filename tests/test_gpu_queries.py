@@ -301,3 +301,29 @@ def test_camera_rays_packet_path_bit_exact(gpu, oracle):
     np.testing.assert_array_equal(_np(fi), of)
     np.testing.assert_array_equal(_np(pos), op)
     assert (of >= 0).mean() > 0.05 and (of < 0).any()  # the sphere fills ~11% of these frames
+
+
+def test_valu_probe_counts_and_times(gpu):
+    """cubvh_valu_probe runs 8 workgroups of 4 waves per CU (8 waves on each SIMD), each
+    lane 16 fma chains: its instruction count is CUs x 8 x 4 x iters x 16 and, after a
+    first call that pays the one-time launch costs, its time grows with iters; the
+    SGPR-operand form is not faster than the VGPR form."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    n_cu = torch.cuda.get_device_properties(gpu).multi_processor_count
+    out = {}
+    for form in (0, 1):
+        warm_wi, warm_ms = ctypes.c_double(0.0), ctypes.c_float(0.0)
+        _lib.check(lib.cubvh_valu_probe(gpu.index, form, 2000, ctypes.byref(warm_wi), ctypes.byref(warm_ms)))
+        for iters in (2000, 8000):
+            wi = ctypes.c_double(0.0)
+            ms = ctypes.c_float(0.0)
+            _lib.check(lib.cubvh_valu_probe(gpu.index, form, iters, ctypes.byref(wi), ctypes.byref(ms)))
+            assert wi.value == n_cu * 8 * 4 * iters * 16
+            assert ms.value > 0
+            out[form, iters] = ms.value
+        assert out[form, 8000] > 2.0 * out[form, 2000]
+    assert out[1, 8000] > 0.9 * out[0, 8000]
