@@ -200,6 +200,28 @@ def test_sdf_raystab_bit_exact_closed_and_soup(gpu, oracle, ico4):
     np.testing.assert_array_equal(_np(fi), of)
 
 
+def test_sdf_raystab_threshold_sizes_and_empty_lists(gpu, oracle, ico4):
+    """The raystab passes at the packet path's threshold (2,048 and 2,049 queries: lists
+    shorter than a workgroup) and with every query outside the mesh (line 0 lists no
+    query, so every line batch and compaction runs on an empty list): bit-exact with the
+    oracle."""
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    for n, seed in ((2048, 21), (2049, 22)):
+        p = S.uniform_points(n, seed=seed, lo=-0.7, hi=0.7)
+        d, fi, _ = bvh.signed_distance(torch.from_numpy(p).to(gpu), mode="raystab")
+        od, of, _ = oracle.sdf_brute(v, f, p, 1, return_uvw=False)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        assert (od < 0).any() and (od > 0).any()
+    far = S.uniform_points(4096, seed=23, lo=2.0, hi=3.0)
+    d, fi, _ = bvh.signed_distance(torch.from_numpy(far).to(gpu), mode="raystab")
+    od, of, _ = oracle.sdf_brute(v, f, far, 1, return_uvw=False)
+    np.testing.assert_array_equal(_np(d), od)
+    np.testing.assert_array_equal(_np(fi), of)
+    assert (od > 0).all()
+
+
 def test_state_dict_roundtrip_on_gpu(gpu, oracle, ico4):
     """test/state_dict.py:96-112: save, load, from_state_dict give identical results;
     the reference's own layout loads too."""
