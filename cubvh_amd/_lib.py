@@ -62,6 +62,12 @@ SIGNATURES = {
         [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
          c_void_p, c_void_p],
     ),
+    "cubvh_vertex_corners": (c_int, [c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "cubvh_udf_backward_with_corners": (
+        c_int,
+        [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "cubvh_hashtable_create": (c_int, [c_int32, POINTER(c_void_p)]),
     "cubvh_hashtable_destroy": (None, [c_void_p]),
     "cubvh_hashtable_set_num_dims": (c_int, [c_void_p, c_int32]),

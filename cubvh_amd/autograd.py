@@ -21,11 +21,49 @@ from . import _lib
 from .api import cuBVH, _query_tensor, _stream_of
 
 
+def vertex_corner_table(owner, faces, n_vertices):
+    """The vertex -> corners table of a mesh's device faces (cubvh_vertex_corners),
+    built once and kept on `owner` (the BVH or forest whose faces never change) per
+    device and vertex count: (corners [3F], begin [V], end [V]) uint32 tensors, as int32."""
+    key = (faces.device, int(faces.data_ptr()), int(faces.shape[0]), int(n_vertices))
+    table = getattr(owner, "_corner_table", None)
+    if table is not None and table[0] == key:
+        return table[1]
+    dev = faces.device
+    corners = torch.empty(3 * faces.shape[0], dtype=torch.int32, device=dev)
+    begin = torch.empty(n_vertices, dtype=torch.int32, device=dev)
+    end = torch.empty(n_vertices, dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.cubvh_vertex_corners(dev.index, faces.data_ptr(), faces.shape[0], n_vertices, corners.data_ptr(),
+                                        begin.data_ptr(), end.data_ptr(), _stream_of(faces)))
+    owner._corner_table = (key, (corners, begin, end))
+    return corners, begin, end
+
+
+def udf_backward(points, face_id, uvw, vertices, faces, g, need_p, need_v, table=None):
+    """Point and vertex gradients of sum(g * distance) (cubvh_udf_backward, or
+    cubvh_udf_backward_with_corners when the mesh's vertex -> corners table is given)."""
+    dev = points.device
+    gp = torch.empty_like(points) if need_p else None
+    gv = torch.zeros_like(vertices) if need_v else None
+    lib = _lib.load()
+    args = [dev.index, points.shape[0], g.data_ptr(), points.data_ptr(), face_id.data_ptr(), uvw.data_ptr(),
+            vertices.data_ptr(), vertices.shape[0], faces.data_ptr(), faces.shape[0],
+            gp.data_ptr() if gp is not None else None, gv.data_ptr() if gv is not None else None]
+    if table is not None and need_v:
+        _lib.check(lib.cubvh_udf_backward_with_corners(*args, table[0].data_ptr(), table[1].data_ptr(),
+                                                       table[2].data_ptr(), _stream_of(points)))
+    else:
+        _lib.check(lib.cubvh_udf_backward(*args, _stream_of(points)))
+    return gp, gv
+
+
 class _UnsignedDistance(torch.autograd.Function):
     @staticmethod
     def forward(ctx, points, vertices, bvh, faces):
         dist, face_id, uvw = bvh.unsigned_distance(points, return_uvw=True)
         ctx.save_for_backward(points, face_id, uvw, vertices, faces)
+        ctx.bvh = bvh
         ctx.mark_non_differentiable(face_id, uvw)
         return dist, face_id, uvw
 
@@ -36,18 +74,8 @@ class _UnsignedDistance(torch.autograd.Function):
         if not (need_p or need_v):
             return None, None, None, None
         g = g_dist.float().contiguous()
-        dev = points.device
-        gp = torch.empty_like(points) if need_p else None
-        gv = torch.zeros_like(vertices) if need_v else None
-        lib = _lib.load()
-        _lib.check(
-            lib.cubvh_udf_backward(
-                dev.index, points.shape[0], g.data_ptr(), points.data_ptr(), face_id.data_ptr(), uvw.data_ptr(),
-                vertices.data_ptr(), vertices.shape[0], faces.data_ptr(), faces.shape[0],
-                gp.data_ptr() if gp is not None else None, gv.data_ptr() if gv is not None else None,
-                _stream_of(points),
-            )
-        )
+        table = vertex_corner_table(ctx.bvh, faces, vertices.shape[0]) if need_v else None
+        gp, gv = udf_backward(points, face_id, uvw, vertices, faces, g, need_p, need_v, table)
         return gp, gv, None, None
 
 

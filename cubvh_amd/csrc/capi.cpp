@@ -937,11 +937,12 @@ int cubvh_floodfill(int device, const uint8_t* grid, int32_t B, int32_t H, int32
     CUBVH_GUARD_END
 }
 
-int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,
-                       const int64_t* face_id, const float* uvw, const float* vertices, int64_t n_vertices,
-                       const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
-                       void* stream) {
-    CUBVH_GUARD_BEGIN
+namespace {
+
+int udf_backward_impl(int device, int64_t n, const float* grad_dist, const float* positions, const int64_t* face_id,
+                      const float* uvw, const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
+                      float* grad_points, float* grad_vertices, const uint32_t* corners, const uint32_t* corner_begin,
+                      const uint32_t* corner_end, void* stream) {
     if (n < 0) return fail(CUBVH_ERR_ARG, "negative query count");
     if (n == 0) return CUBVH_OK;
     if (!grad_dist || !positions || !face_id || !uvw || !vertices || !faces)
@@ -959,16 +960,64 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
     if (grad_vertices) {
         int rc = ensure_pool(device);
         if (rc != CUBVH_OK) return rc;
-        hipError_t e = hipMallocAsync(&ws, udf_backward_workspace_bytes(n, n_vertices, n_faces, grad_points != nullptr), hs);
+        hipError_t e = hipMallocAsync(
+            &ws, udf_backward_workspace_bytes(n, n_vertices, n_faces, grad_points != nullptr, corners != nullptr), hs);
         if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(backward workspace)");
     }
     hipError_t e = launch_udf_backward(n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
-                                       grad_points, grad_vertices, ws, hs);
+                                       grad_points, grad_vertices, corners, corner_begin, corner_end, ws, hs);
     if (ws) {
         hipError_t e2 = hipFreeAsync(ws, hs);
         if (e == hipSuccess) e = e2;
     }
     if (e != hipSuccess) return fail_hip(e, "backward kernel launch");
+    return CUBVH_OK;
+}
+
+}  // namespace
+
+int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,
+                       const int64_t* face_id, const float* uvw, const float* vertices, int64_t n_vertices,
+                       const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
+                       void* stream) {
+    CUBVH_GUARD_BEGIN
+    return udf_backward_impl(device, n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
+                             grad_points, grad_vertices, nullptr, nullptr, nullptr, stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_udf_backward_with_corners(int device, int64_t n, const float* grad_dist, const float* positions,
+                                    const int64_t* face_id, const float* uvw, const float* vertices,
+                                    int64_t n_vertices, const int32_t* faces, int64_t n_faces, float* grad_points,
+                                    float* grad_vertices, const uint32_t* corners, const uint32_t* corner_begin,
+                                    const uint32_t* corner_end, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!corners || !corner_begin || !corner_end) return fail(CUBVH_ERR_ARG, "null vertex corner table");
+    return udf_backward_impl(device, n, grad_dist, positions, face_id, uvw, vertices, n_vertices, faces, n_faces,
+                             grad_points, grad_vertices, corners, corner_begin, corner_end, stream);
+    CUBVH_GUARD_END
+}
+
+int cubvh_vertex_corners(int device, const int32_t* faces, int64_t n_faces, int64_t n_vertices, uint32_t* corners,
+                         uint32_t* corner_begin, uint32_t* corner_end, void* stream) {
+    CUBVH_GUARD_BEGIN
+    if (!faces || !corners || !corner_begin || !corner_end) return fail(CUBVH_ERR_ARG, "null buffer");
+    if (n_vertices <= 0 || n_faces <= 0) return fail(CUBVH_ERR_ARG, "empty mesh");
+    if (n_vertices >= ((int64_t)1 << 31) || 3 * n_faces >= ((int64_t)1 << 31))
+        return fail(CUBVH_ERR_ARG, "mesh too large (vertices and 3 x faces below 2^31)");
+    if (device < 0 || device >= kMaxDevices) return fail(CUBVH_ERR_ARG, "device ordinal out of range");
+    DeviceScope g(device);
+    if (g.err != hipSuccess) return fail_hip(g.err, "hipSetDevice");
+    int rc = ensure_pool(device);
+    if (rc != CUBVH_OK) return rc;
+    hipStream_t hs = (hipStream_t)stream;
+    void* ws = nullptr;
+    hipError_t e = hipMallocAsync(&ws, vertex_corners_workspace_bytes(n_vertices, n_faces), hs);
+    if (e != hipSuccess) return fail_hip(e, "hipMallocAsync(corner sort workspace)");
+    e = launch_vertex_corners(faces, n_faces, n_vertices, corners, corner_begin, corner_end, ws, hs);
+    hipError_t e2 = hipFreeAsync(ws, hs);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail_hip(e, "vertex corner table");
     return CUBVH_OK;
     CUBVH_GUARD_END
 }

@@ -14,14 +14,20 @@
 //   1. query_terms  : g n per query (the point gradient), and its face as a sort key
 //   2. radix sort   : queries grouped by face, stable (ascending query index inside
 //                     a face)
-//   3. face_ranges  : each face's range of the sorted queries, and its number of
-//                     chunks of kChunk queries counted from the face's first query
-//   4. scan         : the chunks' positions (exclusive sum over the faces)
+//   3. run_bounds   : each face's range [begin, end) of the sorted queries, from the
+//                     run boundaries of the sorted keys (one pass; absent faces 0, 0)
+//   4. face_chunks  : each face's number of chunks of kChunk queries, counted from the
+//                     face's first query; a scan gives the chunks' positions, and
+//                     chunk_faces writes each chunk's face
 //   5. chunk_sums   : one lane per chunk adds -w_k g n over its queries in order
 //   6. face_totals  : one lane per face adds its chunks' sums in order
-//   7. radix sort   : the face corners (3 f + k) grouped by vertex, stable
-//   8. vertex_sums  : one lane per vertex adds its corners' face totals in corner
+//   7. vertex_sums  : one lane per vertex adds its corners' face totals in corner
 //                     order and adds the total to grad_vertices
+// Step 7 reads the vertex -> corners table (the corners 3 f + k of every face grouped
+// by vertex, ascending inside a vertex: a stable radix sort of the corners by vertex
+// and run_bounds over it).  It depends only on the faces, so a caller that runs many
+// backward passes over one mesh builds it once (launch_vertex_corners); otherwise
+// each call builds it in its workspace.
 // Queries per face are very uneven (far points all reach the few faces that stick out
 // most: on the jittered level-6 sphere with 2^20 N(0, 0.4^2) points, 45% of the faces get
 // none and one gets 2,867), so a face's queries are split into chunks that lanes sum
@@ -79,59 +85,60 @@ __global__ __launch_bounds__(256) void query_terms(int64_t n, const float* __res
     }
 }
 
-// First position in sorted[0, n) whose value is >= x.
-__device__ __forceinline__ int64_t lower_bound(const uint32_t* __restrict__ sorted, int64_t n, uint32_t x) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (sorted[mid] < x) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 constexpr int kChunk = 32;  // queries per chunk of a face's sorted queries
 
-// Per face: its range [start, start + count) of the sorted queries and its chunk
-// count.  One lane per face, two binary searches in the sorted keys.  nchunk has
-// n_faces + 1 entries, the last one 0 (the scan's total).
-__global__ __launch_bounds__(256) void face_ranges(int64_t n, int64_t n_faces, const uint32_t* __restrict__ skey,
-                                                   uint32_t* __restrict__ start, uint32_t* __restrict__ nchunk) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f > n_faces) return;
-    if (f == n_faces) {
-        nchunk[f] = 0;
-        return;
-    }
-    const int64_t lo = lower_bound(skey, n, (uint32_t)f);
-    const int64_t hi = lower_bound(skey, n, (uint32_t)f + 1u);
-    start[f] = (uint32_t)lo;
-    nchunk[f] = (uint32_t)((hi - lo + kChunk - 1) / kChunk);
+__global__ __launch_bounds__(256) void grad_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
-// Lane per chunk t < cbase[n_faces]: the face f with cbase[f] <= t < cbase[f + 1],
-// chunk k = t - cbase[f] covers the face's queries kChunk k .. kChunk (k + 1) - 1 (of
-// its count); their -w_k g n summed in order go to part[9 t ..].
-__global__ __launch_bounds__(256) void chunk_sums(int64_t n, int64_t n_faces, int64_t max_chunks,
-                                                  const uint32_t* __restrict__ skey,
-                                                  const uint32_t* __restrict__ sval, const uint32_t* __restrict__ start,
-                                                  const uint32_t* __restrict__ cbase, const float* __restrict__ gn,
-                                                  const float* __restrict__ uvw, float* __restrict__ part) {
+// Runs of equal keys in sorted[0, n): for each key k < n_keys present, begin[k] = its
+// first position and end[k] = one past its last (begin and end must hold 0 for keys
+// that are absent; keys >= n_keys are sentinels and ignored).
+__global__ __launch_bounds__(256) void run_bounds(int64_t n, const uint32_t* __restrict__ sorted, int64_t n_keys,
+                                                  uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = sorted[i];
+    if ((int64_t)k >= n_keys) return;
+    if (i == 0 || sorted[i - 1] != k) begin[k] = (uint32_t)i;
+    if (i == n - 1 || sorted[i + 1] != k) end[k] = (uint32_t)(i + 1);
+}
+
+// Per face: its number of chunks of kChunk sorted queries.  nchunk has n_faces + 1
+// entries, the last one 0 (the scan's total).
+__global__ __launch_bounds__(256) void face_chunks(int64_t n_faces, const uint32_t* __restrict__ begin,
+                                                   const uint32_t* __restrict__ end, uint32_t* __restrict__ nchunk) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f > n_faces) return;
+    nchunk[f] = f == n_faces ? 0u : (end[f] - begin[f] + kChunk - 1) / kChunk;
+}
+
+// Per face: its chunks cbase[f] .. cbase[f + 1] - 1 belong to it (cface[t] = f).
+__global__ __launch_bounds__(256) void chunk_faces(int64_t n_faces, int64_t max_chunks,
+                                                   const uint32_t* __restrict__ cbase, uint32_t* __restrict__ cface) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_faces) return;
+    const int64_t e = min((int64_t)cbase[f + 1], max_chunks);
+    for (int64_t t = cbase[f]; t < e; ++t) cface[t] = (uint32_t)f;
+}
+
+// Lane per chunk t < cbase[n_faces]: chunk k = t - cbase[f] of face f = cface[t] covers
+// the face's sorted queries begin[f] + kChunk k .. (up to end[f]); their -w_k g n summed
+// in order go to part[9 t ..].
+__global__ __launch_bounds__(256) void chunk_sums(int64_t n_faces, int64_t max_chunks, const uint32_t* __restrict__ sval,
+                                                  const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
+                                                  const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ cface,
+                                                  const float* __restrict__ gn, const float* __restrict__ uvw,
+                                                  float* __restrict__ part) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = cbase[n_faces];
     if (t >= total || t >= max_chunks) return;
-    // face: last f with cbase[f] <= t (faces without queries have cbase[f] == cbase[f + 1])
-    int64_t lo = 0, hi = n_faces - 1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (cbase[mid] <= (uint32_t)t) lo = mid;
-        else hi = mid - 1;
-    }
-    const int64_t f = lo;
-    const int64_t k = t - cbase[f];
-    const int64_t j0 = (int64_t)start[f] + kChunk * k;
+    const int64_t f = cface[t];
+    const int64_t j0 = (int64_t)begin[f] + kChunk * (t - (int64_t)cbase[f]);
+    const int64_t j1 = min(j0 + kChunk, (int64_t)end[f]);
     float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t j = j0; j < j0 + kChunk && j < n && skey[j] == (uint32_t)f; ++j) {
+    for (int64_t j = j0; j < j1; ++j) {
         const int64_t q = sval[j];
         const float x = gn[3 * q], y = gn[3 * q + 1], z = gn[3 * q + 2];
 #pragma unroll
@@ -174,17 +181,17 @@ __global__ __launch_bounds__(256) void corner_keys(int64_t n_corners, const int3
     val[s] = (uint32_t)s;
 }
 
-// Lane per vertex: adds the sums of its corners (corner index ascending) to gv.
-__global__ __launch_bounds__(256) void vertex_sums(int64_t n_corners, int64_t n_vertices,
-                                                   const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+// Lane per vertex: adds the sums of its corners corners[begin[v] .. end[v]) (corner
+// index ascending) to gv.
+__global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, const uint32_t* __restrict__ corners,
+                                                   const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
                                                    const float* __restrict__ acc, float* __restrict__ gv) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_vertices) return;
-    const int64_t lo = lower_bound(skey, n_corners, (uint32_t)v);
-    const int64_t hi = lower_bound(skey, n_corners, (uint32_t)v + 1u);
+    const int64_t lo = begin[v], hi = end[v];
     float x = 0.0f, y = 0.0f, z = 0.0f;
     for (int64_t j = lo; j < hi; ++j) {
-        const int64_t s = sval[j];
+        const int64_t s = corners[j];
         x += acc[3 * s];
         y += acc[3 * s + 1];
         z += acc[3 * s + 2];
@@ -224,14 +231,14 @@ size_t scan_temp_bytes(int64_t m) {
 // that has queries.
 int64_t max_chunks(int64_t n, int64_t n_faces) { return n / dev::kChunk + std::min(n, n_faces); }
 
-struct BwdLayout {
-    size_t gn = 0, k0 = 0, v0 = 0, k1 = 0, v1 = 0, acc = 0, c0 = 0, cv0 = 0, c1 = 0, cv1 = 0, tmp = 0, total = 0;
-    size_t start = 0, nchunk = 0, cbase = 0, part = 0;
-    size_t tmp_bytes = 0;
+// Workspace of launch_vertex_corners: corner keys and values before the sort, the
+// sorted keys, and the sort's temporary storage.
+struct CornerLayout {
+    size_t c0 = 0, cv0 = 0, c1 = 0, tmp = 0, total = 0, tmp_bytes = 0;
 };
 
-BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_gn) {
-    BwdLayout L;
+CornerLayout corner_layout(int64_t n_vertices, int64_t n_faces) {
+    CornerLayout L;
     size_t o = 0;
     auto take = [&](size_t bytes) {
         size_t at = o;
@@ -239,22 +246,49 @@ BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_g
         return at;
     };
     const int64_t nc = 3 * n_faces;
+    L.c0 = take(4 * (size_t)nc);
+    L.cv0 = take(4 * (size_t)nc);
+    L.c1 = take(4 * (size_t)nc);
+    L.tmp_bytes = sort_temp_bytes(nc, key_bits(n_vertices));
+    L.tmp = take(L.tmp_bytes);
+    L.total = o;
+    return L;
+}
+
+struct BwdLayout {
+    size_t gn = 0, k0 = 0, v0 = 0, k1 = 0, v1 = 0, acc = 0, tmp = 0, total = 0;
+    size_t fbeg = 0, fend = 0, nchunk = 0, cbase = 0, cface = 0, part = 0;
+    size_t corners = 0, vbeg = 0, vend = 0, corner_ws = 0;
+    size_t tmp_bytes = 0;
+};
+
+BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_gn, bool need_tables) {
+    BwdLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += al256(bytes);
+        return at;
+    };
     if (need_gn) L.gn = take(12 * (size_t)n);
     L.k0 = take(4 * (size_t)n);
     L.v0 = take(4 * (size_t)n);
     L.k1 = take(4 * (size_t)n);
     L.v1 = take(4 * (size_t)n);
     L.acc = take(36 * (size_t)n_faces);
-    L.start = take(4 * (size_t)n_faces);
+    L.fbeg = take(4 * (size_t)n_faces);
+    L.fend = take(4 * (size_t)n_faces);
     L.nchunk = take(4 * (size_t)(n_faces + 1));
     L.cbase = take(4 * (size_t)(n_faces + 1));
+    L.cface = take(4 * (size_t)max_chunks(n, n_faces));
     L.part = take(36 * (size_t)max_chunks(n, n_faces));
-    L.c0 = take(4 * (size_t)nc);
-    L.cv0 = take(4 * (size_t)nc);
-    L.c1 = take(4 * (size_t)nc);
-    L.cv1 = take(4 * (size_t)nc);
-    L.tmp_bytes = std::max(std::max(sort_temp_bytes(n, key_bits(n_faces)), sort_temp_bytes(nc, key_bits(n_vertices))),
-                           scan_temp_bytes(n_faces + 1));
+    if (need_tables) {
+        L.corners = take(12 * (size_t)n_faces);
+        L.vbeg = take(4 * (size_t)n_vertices);
+        L.vend = take(4 * (size_t)n_vertices);
+        L.corner_ws = take(corner_layout(n_vertices, n_faces).total);
+    }
+    L.tmp_bytes = std::max(sort_temp_bytes(n, key_bits(n_faces)), scan_temp_bytes(n_faces + 1));
     L.tmp = take(L.tmp_bytes);
     L.total = o;
     return L;
@@ -262,14 +296,41 @@ BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_g
 
 }  // namespace
 
-size_t udf_backward_workspace_bytes(int64_t n, int64_t n_vertices, int64_t n_faces, bool have_grad_points) {
-    return bwd_layout(n, n_vertices, n_faces, !have_grad_points).total;
+size_t vertex_corners_workspace_bytes(int64_t n_vertices, int64_t n_faces) {
+    return corner_layout(n_vertices, n_faces).total;
+}
+
+hipError_t launch_vertex_corners(const int32_t* faces, int64_t n_faces, int64_t n_vertices, uint32_t* corners,
+                                 uint32_t* begin, uint32_t* end, void* ws, hipStream_t stream) {
+    if (n_vertices <= 0) return hipSuccess;
+    const int64_t nc = 3 * n_faces;
+    const unsigned gv = (unsigned)((n_vertices + 255) / 256);
+    hipLaunchKernelGGL(dev::grad_fill_u32, dim3(gv), dim3(256), 0, stream, begin, n_vertices, 0u);
+    hipLaunchKernelGGL(dev::grad_fill_u32, dim3(gv), dim3(256), 0, stream, end, n_vertices, 0u);
+    if (nc <= 0) return hipGetLastError();
+    const CornerLayout L = corner_layout(n_vertices, n_faces);
+    char* w = (char*)ws;
+    uint32_t *c0 = (uint32_t*)(w + L.c0), *cv0 = (uint32_t*)(w + L.cv0), *c1 = (uint32_t*)(w + L.c1);
+    hipLaunchKernelGGL(dev::corner_keys, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, nc, faces, n_vertices,
+                       c0, cv0);
+    size_t tb = L.tmp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(w + L.tmp), tb, c0, c1, cv0, corners, (int)nc, 0,
+                                                      key_bits(n_vertices), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::run_bounds, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, nc, c1, n_vertices,
+                       begin, end);
+    return hipGetLastError();
+}
+
+size_t udf_backward_workspace_bytes(int64_t n, int64_t n_vertices, int64_t n_faces, bool have_grad_points,
+                                    bool have_tables) {
+    return bwd_layout(n, n_vertices, n_faces, !have_grad_points, !have_tables).total;
 }
 
 hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* positions, const int64_t* face_id,
                                const float* uvw, const float* vertices, int64_t n_vertices, const int32_t* faces,
-                               int64_t n_faces, float* grad_points, float* grad_vertices, void* ws,
-                               hipStream_t stream) {
+                               int64_t n_faces, float* grad_points, float* grad_vertices, const uint32_t* corners,
+                               const uint32_t* corner_begin, const uint32_t* corner_end, void* ws, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const unsigned gq = (unsigned)((n + 255) / 256);
     if (!grad_vertices) {
@@ -279,42 +340,52 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
                                (uint32_t*)nullptr);
         return hipGetLastError();
     }
-    const BwdLayout L = bwd_layout(n, n_vertices, n_faces, grad_points == nullptr);
+    const bool have_tables = corners && corner_begin && corner_end;
+    const BwdLayout L = bwd_layout(n, n_vertices, n_faces, grad_points == nullptr, !have_tables);
     char* w = (char*)ws;
     float* gn = grad_points ? grad_points : (float*)(w + L.gn);
     uint32_t *k0 = (uint32_t*)(w + L.k0), *v0 = (uint32_t*)(w + L.v0);
     uint32_t *k1 = (uint32_t*)(w + L.k1), *v1 = (uint32_t*)(w + L.v1);
     float* acc = (float*)(w + L.acc);
-    uint32_t *c0 = (uint32_t*)(w + L.c0), *cv0 = (uint32_t*)(w + L.cv0);
-    uint32_t *c1 = (uint32_t*)(w + L.c1), *cv1 = (uint32_t*)(w + L.cv1);
     void* tmp = w + L.tmp;
-    const int64_t nc = 3 * n_faces;
+    hipError_t e;
+    if (!have_tables) {
+        corners = (const uint32_t*)(w + L.corners);
+        corner_begin = (const uint32_t*)(w + L.vbeg);
+        corner_end = (const uint32_t*)(w + L.vend);
+        e = launch_vertex_corners(faces, n_faces, n_vertices, (uint32_t*)(w + L.corners), (uint32_t*)(w + L.vbeg),
+                                  (uint32_t*)(w + L.vend), w + L.corner_ws, stream);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(dev::query_terms, dim3(gq), dim3(256), 0, stream, n, grad_dist, positions, face_id, uvw,
                        vertices, faces, n_vertices, n_faces, gn, k0, v0);
     size_t tb = L.tmp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)n, 0, key_bits(n_faces), stream);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)n, 0, key_bits(n_faces), stream);
     if (e != hipSuccess) return e;
-    uint32_t* start = (uint32_t*)(w + L.start);
+    uint32_t* fbeg = (uint32_t*)(w + L.fbeg);
+    uint32_t* fend = (uint32_t*)(w + L.fend);
     uint32_t* nchunk = (uint32_t*)(w + L.nchunk);
     uint32_t* cbase = (uint32_t*)(w + L.cbase);
+    uint32_t* cface = (uint32_t*)(w + L.cface);
     float* part = (float*)(w + L.part);
     const int64_t mc = max_chunks(n, n_faces);
-    hipLaunchKernelGGL(dev::face_ranges, dim3((unsigned)((n_faces + 1 + 255) / 256)), dim3(256), 0, stream, n, n_faces, k1,
-                       start, nchunk);
+    const unsigned gf = (unsigned)((n_faces + 1 + 255) / 256);
+    if (n_faces > 0) {
+        hipLaunchKernelGGL(dev::grad_fill_u32, dim3(gf), dim3(256), 0, stream, fbeg, n_faces, 0u);
+        hipLaunchKernelGGL(dev::grad_fill_u32, dim3(gf), dim3(256), 0, stream, fend, n_faces, 0u);
+    }
+    hipLaunchKernelGGL(dev::run_bounds, dim3(gq), dim3(256), 0, stream, n, k1, n_faces, fbeg, fend);
+    hipLaunchKernelGGL(dev::face_chunks, dim3(gf), dim3(256), 0, stream, n_faces, fbeg, fend, nchunk);
     tb = L.tmp_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, nchunk, cbase, (int)(n_faces + 1), stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dev::chunk_sums, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n, n_faces, mc, k1, v1,
-                       start, cbase, gn, uvw, part);
+    hipLaunchKernelGGL(dev::chunk_faces, dim3(gf), dim3(256), 0, stream, n_faces, mc, cbase, cface);
+    hipLaunchKernelGGL(dev::chunk_sums, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc, v1, fbeg,
+                       fend, cbase, cface, gn, uvw, part);
     hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc, cbase,
                        part, acc);
-    hipLaunchKernelGGL(dev::corner_keys, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, nc, faces, n_vertices,
-                       c0, cv0);
-    tb = L.tmp_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, c0, c1, cv0, cv1, (int)nc, 0, key_bits(n_vertices), stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dev::vertex_sums, dim3((unsigned)((n_vertices + 255) / 256)), dim3(256), 0, stream, nc,
-                       n_vertices, c1, cv1, acc, grad_vertices);
+    hipLaunchKernelGGL(dev::vertex_sums, dim3((unsigned)((n_vertices + 255) / 256)), dim3(256), 0, stream, n_vertices,
+                       corners, corner_begin, corner_end, acc, grad_vertices);
     return hipGetLastError();
 }
 

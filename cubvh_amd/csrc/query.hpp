@@ -71,11 +71,19 @@ hipError_t launch_floodfill(const uint8_t* grid, int32_t B, int32_t H, int32_t W
 
 // Backward of the unsigned distance (grad.hip).  With grad_vertices, `ws` holds
 // udf_backward_workspace_bytes(...) bytes of device scratch (sorts and per-face sums);
-// point gradients alone need none.
-size_t udf_backward_workspace_bytes(int64_t n, int64_t n_vertices, int64_t n_faces, bool have_grad_points);
+// point gradients alone need none.  corners / corner_begin / corner_end: the mesh's
+// vertex -> corners table (launch_vertex_corners), or all null to build it in `ws`.
+size_t udf_backward_workspace_bytes(int64_t n, int64_t n_vertices, int64_t n_faces, bool have_grad_points,
+                                    bool have_tables);
 hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* positions, const int64_t* face_id,
                                const float* uvw, const float* vertices, int64_t n_vertices, const int32_t* faces,
-                               int64_t n_faces, float* grad_points, float* grad_vertices, void* ws,
-                               hipStream_t stream);
+                               int64_t n_faces, float* grad_points, float* grad_vertices, const uint32_t* corners,
+                               const uint32_t* corner_begin, const uint32_t* corner_end, void* ws, hipStream_t stream);
+// The vertex -> corners table of a mesh: corners (3 n_faces) = the corners 3 f + k grouped
+// by vertex, ascending inside a vertex (corners of out-of-range indices last); vertex v
+// owns corners[begin[v] .. end[v]).  ws holds vertex_corners_workspace_bytes(...) bytes.
+size_t vertex_corners_workspace_bytes(int64_t n_vertices, int64_t n_faces);
+hipError_t launch_vertex_corners(const int32_t* faces, int64_t n_faces, int64_t n_vertices, uint32_t* corners,
+                                 uint32_t* begin, uint32_t* end, void* ws, hipStream_t stream);
 
 }  // namespace cubvh_amd

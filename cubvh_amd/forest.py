@@ -163,22 +163,22 @@ class _BatchedUDF(torch.autograd.Function):
         dist, face, uvw = forest.unsigned_distance(points, offsets, True, tuning, check=False)
         gface = face + face_base_per_query
         ctx.save_for_backward(points, gface, uvw, vertices, faces)
+        ctx.forest = forest
         ctx.mark_non_differentiable(face, uvw)
         return dist, face, uvw
 
     @staticmethod
     def backward(ctx, g_dist, g_face, g_uvw):
+        from .autograd import udf_backward, vertex_corner_table
+
         points, gface, uvw, vertices, faces = ctx.saved_tensors
         need_p, need_v = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_p or need_v):
+            return None, None, None, None, None, None, None
         g = g_dist.float().contiguous()
-        gp = torch.empty_like(points) if need_p else None
-        gv = torch.zeros_like(vertices) if need_v else None
-        lib = _lib.load()
-        _lib.check(lib.cubvh_udf_backward(
-            points.device.index, points.shape[0], g.data_ptr(), points.data_ptr(), gface.data_ptr(), uvw.data_ptr(),
-            vertices.data_ptr(), vertices.shape[0], faces.data_ptr(), faces.shape[0],
-            gp.data_ptr() if gp is not None else None, gv.data_ptr() if gv is not None else None,
-            _stream_of(points)))
+        # the concatenated faces never change: their vertex -> corners table is built once
+        table = vertex_corner_table(ctx.forest, faces, vertices.shape[0]) if need_v else None
+        gp, gv = udf_backward(points, gface, uvw, vertices, faces, g, need_p, need_v, table)
         return gp, gv, None, None, None, None, None
 
 

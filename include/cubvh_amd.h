@@ -195,6 +195,25 @@ int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const floa
                        const int32_t* faces, int64_t n_faces, float* grad_points, float* grad_vertices,
                        void* stream);
 
+/* No counterpart in the reference: the vertex -> corners table of a mesh, which the
+ * vertex gradients read and which depends only on the faces.  corners uint32 [3 n_faces]
+ * = the corners 3 f + k grouped by vertex, ascending inside a vertex (corners with an
+ * out-of-range vertex index last); vertex v owns corners[corner_begin[v] ..
+ * corner_end[v]) (uint32 [n_vertices] each; 0, 0 for an unused vertex).  Device buffers
+ * on `device`; allocates stream-ordered scratch for the sort.  3 n_faces < 2^31. */
+int cubvh_vertex_corners(int device, const int32_t* faces, int64_t n_faces, int64_t n_vertices, uint32_t* corners,
+                         uint32_t* corner_begin, uint32_t* corner_end, void* stream);
+
+/* cubvh_udf_backward with the mesh's vertex -> corners table built beforehand by
+ * cubvh_vertex_corners from the same faces (for callers that run many backward passes
+ * over one mesh: each call then sorts only its n face ids).  The same bits as
+ * cubvh_udf_backward. */
+int cubvh_udf_backward_with_corners(int device, int64_t n, const float* grad_dist, const float* positions,
+                                    const int64_t* face_id, const float* uvw, const float* vertices,
+                                    int64_t n_vertices, const int32_t* faces, int64_t n_faces, float* grad_points,
+                                    float* grad_vertices, const uint32_t* corners, const uint32_t* corner_begin,
+                                    const uint32_t* corner_end, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Batched queries over many meshes.  No counterpart in the reference (which takes
  * one mesh per cuBVH and one launch per query call); for callers with many meshes

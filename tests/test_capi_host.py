@@ -436,3 +436,21 @@ def test_valu_probe_argument_checks():
         assert b"iters" in lib.cubvh_last_error()
     assert lib.cubvh_valu_probe(-1, 0, 100, ctypes.byref(wi), ctypes.byref(ms)) == -1
     assert b"device ordinal" in lib.cubvh_last_error()
+
+
+def test_vertex_corner_entry_points_argument_checks():
+    """cubvh_vertex_corners and cubvh_udf_backward_with_corners refuse null buffers and
+    empty meshes before they touch a device."""
+    from cubvh_amd import _lib
+
+    lib = _lib.load()
+    buf = ctypes.c_void_p(1)
+    assert lib.cubvh_vertex_corners(0, None, 10, 10, buf, buf, buf, None) == -1
+    assert b"null buffer" in lib.cubvh_last_error()
+    assert lib.cubvh_vertex_corners(0, buf, 0, 10, buf, buf, buf, None) == -1
+    assert b"empty mesh" in lib.cubvh_last_error()
+    assert lib.cubvh_vertex_corners(0, buf, 1 << 30, 10, buf, buf, buf, None) == -1
+    assert b"too large" in lib.cubvh_last_error()
+    assert lib.cubvh_udf_backward_with_corners(0, 4, buf, buf, buf, buf, buf, 3, buf, 1, None, buf, None, buf, buf,
+                                               None) == -1
+    assert b"corner table" in lib.cubvh_last_error()

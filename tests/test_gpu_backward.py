@@ -149,3 +149,65 @@ def test_backward_bit_exact_vs_oracle_and_repeatable(gpu, oracle, record):
     np.testing.assert_array_equal(outs[0][1].cpu().numpy(), ogv)
     record("backward_level6_2p20", queries=n, vertices=int(len(v)),
            max_abs_vertex_grad=float(np.abs(ogv).max()), point_grad_equal=True, vertex_grad_equal=True)
+
+
+def test_vertex_corner_table_and_backward_with_it(gpu, oracle):
+    """cubvh_vertex_corners equals a stable numpy argsort of the corners by vertex (a
+    corner with an out-of-range index goes last and belongs to no vertex), and the
+    backward with that table gives the same bits as the one that builds it per call."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    v, f = S.jittered_sphere(4, seed=2)
+    nv = len(v)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    fbad = f.copy()
+    fbad[7, 1] = nv + 5  # out of range: the table skips it
+    for faces in (f, fbad):
+        Ft = torch.from_numpy(faces).to(gpu)
+        corners = torch.empty(3 * len(faces), dtype=torch.int32, device=gpu)
+        begin = torch.empty(nv, dtype=torch.int32, device=gpu)
+        end = torch.empty(nv, dtype=torch.int32, device=gpu)
+        _lib.check(lib.cubvh_vertex_corners(gpu.index, Ft.data_ptr(), len(faces), nv, corners.data_ptr(),
+                                            begin.data_ptr(), end.data_ptr(), s))
+        torch.cuda.synchronize()
+        keys = faces.reshape(-1).astype(np.int64)
+        keys = np.where((keys >= 0) & (keys < nv), keys, nv)
+        order = np.argsort(keys, kind="stable")
+        np.testing.assert_array_equal(corners.cpu().numpy(), order.astype(np.int32))
+        sk = keys[order]
+        np.testing.assert_array_equal(begin.cpu().numpy(), np.searchsorted(sk, np.arange(nv), "left"))
+        np.testing.assert_array_equal(end.cpu().numpy(), np.searchsorted(sk, np.arange(nv), "right"))
+    # the backward through the table equals the one building it, bit for bit
+    n = 1 << 16
+    pts = S.normal_points(n, 0.4, seed=5)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    g = torch.linspace(-1, 1, n, device=gpu)
+    corners = torch.empty(3 * len(f), dtype=torch.int32, device=gpu)
+    begin = torch.empty(nv, dtype=torch.int32, device=gpu)
+    end = torch.empty(nv, dtype=torch.int32, device=gpu)
+    _lib.check(lib.cubvh_vertex_corners(gpu.index, Ft.data_ptr(), len(f), nv, corners.data_ptr(), begin.data_ptr(),
+                                        end.data_ptr(), s))
+    outs = []
+    for with_table in (False, True):
+        gp = torch.empty_like(P)
+        gv = torch.zeros_like(Vt)
+        args = [gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(), Vt.data_ptr(), nv,
+                Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr()]
+        if with_table:
+            _lib.check(lib.cubvh_udf_backward_with_corners(*args, corners.data_ptr(), begin.data_ptr(),
+                                                           end.data_ptr(), s))
+        else:
+            _lib.check(lib.cubvh_udf_backward(*args, s))
+        outs.append((gp, gv))
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    assert outs[0][1].abs().sum() > 0
+    ogp, ogv = oracle.udf_backward(v, f, pts, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
+    np.testing.assert_array_equal(outs[1][1].cpu().numpy(), ogv)
