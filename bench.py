@@ -257,7 +257,7 @@ def fwd_bwd_leg(dev, steps, warmup):
             "queries": n, "fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf,
             "fwd_Mq_s": n / tf / 1e6, "fwd_bwd_Mq_s": n / tfb / 1e6, "bwd_kernel_ms": tb * 1e3,
             "roofline_bwd": dict(roofline_block("udf_backward_level6",
-                                                "query_terms + face_sums + corner_keys + vertex_sums + 2 radix sorts",
+                                                "query_terms + face sums + vertex corner table + vertex_sums (3 radix sorts)",
                                                 tb, units=n, algo_bytes_per_unit=168),
                                  time_basis="whole cubvh_udf_backward call")}
 

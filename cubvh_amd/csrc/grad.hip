@@ -349,14 +349,6 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     float* acc = (float*)(w + L.acc);
     void* tmp = w + L.tmp;
     hipError_t e;
-    if (!have_tables) {
-        corners = (const uint32_t*)(w + L.corners);
-        corner_begin = (const uint32_t*)(w + L.vbeg);
-        corner_end = (const uint32_t*)(w + L.vend);
-        e = launch_vertex_corners(faces, n_faces, n_vertices, (uint32_t*)(w + L.corners), (uint32_t*)(w + L.vbeg),
-                                  (uint32_t*)(w + L.vend), w + L.corner_ws, stream);
-        if (e != hipSuccess) return e;
-    }
     hipLaunchKernelGGL(dev::query_terms, dim3(gq), dim3(256), 0, stream, n, grad_dist, positions, face_id, uvw,
                        vertices, faces, n_vertices, n_faces, gn, k0, v0);
     size_t tb = L.tmp_bytes;
@@ -384,6 +376,14 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
                        fend, cbase, cface, gn, uvw, part);
     hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc, cbase,
                        part, acc);
+    if (!have_tables) {
+        corners = (const uint32_t*)(w + L.corners);
+        corner_begin = (const uint32_t*)(w + L.vbeg);
+        corner_end = (const uint32_t*)(w + L.vend);
+        e = launch_vertex_corners(faces, n_faces, n_vertices, (uint32_t*)(w + L.corners), (uint32_t*)(w + L.vbeg),
+                                  (uint32_t*)(w + L.vend), w + L.corner_ws, stream);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(dev::vertex_sums, dim3((unsigned)((n_vertices + 255) / 256)), dim3(256), 0, stream, n_vertices,
                        corners, corner_begin, corner_end, acc, grad_vertices);
     return hipGetLastError();
