@@ -134,6 +134,9 @@ def roofline_block(name, kernel, kernel_s, units=None, algo_bytes_per_unit=None,
         blk["limiter"] = "VALU issue (SQ_INSTS_VALU per second against the measured fp32 fma issue ceiling)"
     elif blk["frac"] is not None and blk["frac"] >= 0.5:
         blk["limiter"] = "HBM"
+    elif vf is not None and vf >= 0.45:
+        blk["limiter"] = ("VALU issue with latency (between the SGPR-operand and the VGPR-operand fma ceilings, "
+                          "which differ by about 2x)")
     elif blk["frac"] is not None:
         blk["limiter"] = "latency (neither VALU issue nor HBM near its peak)"
     return blk
