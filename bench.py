@@ -565,7 +565,7 @@ def run_extras(dev, steps, warmup, out, gpu_build=True):
     _, tu = _time_kernel(lambda: bvhs.unsigned_distance(Q), max(2, steps // 2), 1)
     # the stab rays' kernel: the raystab call minus the unsigned call it starts with
     out["raystab_soup_4M"] = {"Mq_s": Q.shape[0] / t / 1e6, "ms": t * 1e3, "udf_part_ms": tu * 1e3,
-                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_init + stab_first + 4 x (stab_lines + stab_keep) + stab_negate", max(t - tu, 1e-9)),
+                              "roofline": dict(roofline_block("raystab_soup_4M", "stab_init + stab_first + 3 x (stab_lines + stab_keep) + stab_negate", max(t - tu, 1e-9)),
                                                time_basis="raystab call minus the unsigned call")}
 
 
