@@ -18,7 +18,7 @@
 //                     run boundaries of the sorted keys (one pass; absent faces 0, 0)
 //   4. face_chunks  : each face's number of chunks of kChunk queries, counted from the
 //                     face's first query; a scan gives the chunks' positions, and
-//                     chunk_faces writes each chunk's face
+//                     chunk_marks + a max-scan give each chunk its face
 //   5. chunk_sums   : one lane per chunk adds -w_k g n over its queries in order
 //   6. face_totals  : one lane per face adds its chunks' sums in order
 //   7. vertex_sums  : one lane per vertex adds its corners' face totals in corner
@@ -114,16 +114,18 @@ __global__ __launch_bounds__(256) void face_chunks(int64_t n_faces, const uint32
     nchunk[f] = f == n_faces ? 0u : (end[f] - begin[f] + kChunk - 1) / kChunk;
 }
 
-// Per face: its chunks cbase[f] .. cbase[f + 1] - 1 belong to it (cface[t] = f).
-__global__ __launch_bounds__(256) void chunk_faces(int64_t n_faces, int64_t max_chunks,
-                                                   const uint32_t* __restrict__ cbase, uint32_t* __restrict__ cface) {
+// Per face with queries: marks its first chunk with f + 1 (mark holds zeros elsewhere);
+// an inclusive max-scan of the marks then gives every chunk its face + 1, whatever the
+// number of chunks of one face.
+__global__ __launch_bounds__(256) void chunk_marks(int64_t n_faces, int64_t max_chunks,
+                                                   const uint32_t* __restrict__ nchunk,
+                                                   const uint32_t* __restrict__ cbase, uint32_t* __restrict__ mark) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n_faces) return;
-    const int64_t e = min((int64_t)cbase[f + 1], max_chunks);
-    for (int64_t t = cbase[f]; t < e; ++t) cface[t] = (uint32_t)f;
+    if (f >= n_faces || nchunk[f] == 0u || (int64_t)cbase[f] >= max_chunks) return;
+    mark[cbase[f]] = (uint32_t)f + 1u;
 }
 
-// Lane per chunk t < cbase[n_faces]: chunk k = t - cbase[f] of face f = cface[t] covers
+// Lane per chunk t < cbase[n_faces]: chunk k = t - cbase[f] of face f = cface[t] - 1 covers
 // the face's sorted queries begin[f] + kChunk k .. (up to end[f]); their -w_k g n summed
 // in order go to part[9 t ..].
 __global__ __launch_bounds__(256) void chunk_sums(int64_t n_faces, int64_t max_chunks, const uint32_t* __restrict__ sval,
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(256) void chunk_sums(int64_t n_faces, int64_t max_c
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = cbase[n_faces];
     if (t >= total || t >= max_chunks) return;
-    const int64_t f = cface[t];
+    const int64_t f = (int64_t)cface[t] - 1;
     const int64_t j0 = (int64_t)begin[f] + kChunk * (t - (int64_t)cbase[f]);
     const int64_t j1 = min(j0 + kChunk, (int64_t)end[f]);
     float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -221,6 +223,12 @@ size_t sort_temp_bytes(int64_t m, int bits) {
     return tmp;
 }
 
+size_t max_scan_temp_bytes(int64_t m) {
+    size_t tmp = 0;
+    hipcub::DeviceScan::InclusiveScan(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr, hipcub::Max(), (int)m);
+    return tmp;
+}
+
 size_t scan_temp_bytes(int64_t m) {
     size_t tmp = 0;
     hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)m);
@@ -257,7 +265,7 @@ CornerLayout corner_layout(int64_t n_vertices, int64_t n_faces) {
 
 struct BwdLayout {
     size_t gn = 0, k0 = 0, v0 = 0, k1 = 0, v1 = 0, acc = 0, tmp = 0, total = 0;
-    size_t fbeg = 0, fend = 0, nchunk = 0, cbase = 0, cface = 0, part = 0;
+    size_t fbeg = 0, fend = 0, nchunk = 0, cbase = 0, mark = 0, cface = 0, part = 0;
     size_t corners = 0, vbeg = 0, vend = 0, corner_ws = 0;
     size_t tmp_bytes = 0;
 };
@@ -280,6 +288,7 @@ BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_g
     L.fend = take(4 * (size_t)n_faces);
     L.nchunk = take(4 * (size_t)(n_faces + 1));
     L.cbase = take(4 * (size_t)(n_faces + 1));
+    L.mark = take(4 * (size_t)max_chunks(n, n_faces));
     L.cface = take(4 * (size_t)max_chunks(n, n_faces));
     L.part = take(36 * (size_t)max_chunks(n, n_faces));
     if (need_tables) {
@@ -288,7 +297,8 @@ BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_g
         L.vend = take(4 * (size_t)n_vertices);
         L.corner_ws = take(corner_layout(n_vertices, n_faces).total);
     }
-    L.tmp_bytes = std::max(sort_temp_bytes(n, key_bits(n_faces)), scan_temp_bytes(n_faces + 1));
+    L.tmp_bytes = std::max(std::max(sort_temp_bytes(n, key_bits(n_faces)), scan_temp_bytes(n_faces + 1)),
+                           max_scan_temp_bytes(max_chunks(n, n_faces)));
     L.tmp = take(L.tmp_bytes);
     L.total = o;
     return L;
@@ -371,7 +381,12 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     tb = L.tmp_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, nchunk, cbase, (int)(n_faces + 1), stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dev::chunk_faces, dim3(gf), dim3(256), 0, stream, n_faces, mc, cbase, cface);
+    uint32_t* mark = (uint32_t*)(w + L.mark);
+    hipLaunchKernelGGL(dev::grad_fill_u32, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, mark, mc, 0u);
+    hipLaunchKernelGGL(dev::chunk_marks, dim3(gf), dim3(256), 0, stream, n_faces, mc, nchunk, cbase, mark);
+    tb = L.tmp_bytes;
+    e = hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, cface, hipcub::Max(), (int)mc, stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::chunk_sums, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc, v1, fbeg,
                        fend, cbase, cface, gn, uvw, part);
     hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc, cbase,

@@ -211,3 +211,36 @@ def test_vertex_corner_table_and_backward_with_it(gpu, oracle):
     assert outs[0][1].abs().sum() > 0
     ogp, ogv = oracle.udf_backward(v, f, pts, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
     np.testing.assert_array_equal(outs[1][1].cpu().numpy(), ogv)
+
+
+def test_backward_one_face_takes_almost_every_query(gpu, oracle):
+    """Skewed faces: one large triangle under 2^17 points (thousands of chunks of one
+    face) and ten small triangles far away; gradients bit-exact against the oracle."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    rng = np.random.default_rng(11)
+    big = np.array([[-2.0, -2.0, 0.0], [4.0, -2.0, 0.0], [-2.0, 4.0, 0.0]], np.float32)
+    far = (rng.uniform(-0.01, 0.01, size=(10, 3, 3)) + np.array([5.0, 5.0, 5.0])).reshape(-1, 3).astype(np.float32)
+    v = np.concatenate([big, far]).astype(np.float32)
+    f = np.arange(len(v), dtype=np.int32).reshape(-1, 3)
+    n = 1 << 17
+    pts = np.concatenate([rng.uniform(-1, 1, size=(n, 2)), rng.uniform(0.05, 0.5, size=(n, 1))], axis=1).astype(np.float32)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    assert (fid == 0).float().mean() > 0.99
+    g = torch.linspace(0.5, 1.5, n, device=gpu)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(Vt)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    _lib.check(lib.cubvh_udf_backward(gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(),
+                                      Vt.data_ptr(), len(v), Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr(), s))
+    torch.cuda.synchronize()
+    ogp, ogv = oracle.udf_backward(v, f, pts, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
+    np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
+    np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
