@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 
+
 #include "bvh.hpp"
 #include "geom.hpp"
 #include "query.hpp"
@@ -894,10 +895,12 @@ constexpr int kStabBatches = 3;
 constexpr uint32_t kStabBatchFirst[kStabBatches + 1] = {1u, 4u, 12u, 32u};
 // idle lanes of a wave take new items once this many of the 64 are idle (or all are)
 constexpr int kRefillLanes = 16;
-// waves per SIMD the persistent stab kernels are compiled for (80 VGPRs); with 12 KB
-// of LDS per 2-wave workgroup, 12 workgroups per CU
+// waves per SIMD the persistent stab kernels are compiled for (80 VGPRs); the stab
+// kernels run one-wave workgroups (19.3 ms per call against 19.6 with two) with 6 KB of
+// LDS each, 4 x kStabWaves of them per CU
 constexpr int kStabWaves = 6;
-constexpr int kStabBlocksPerCu = 2 * kStabWaves;
+constexpr int kStabBlock = 64;
+constexpr int kStabBlocksPerCu = 4 * kStabWaves;
 
 __global__ void stab_init(uint32_t* counts, int64_t n) {
     counts[threadIdx.x] = threadIdx.x == 0 ? (uint32_t)n : 0u;  // 64 threads: the 256-byte header
@@ -1213,14 +1216,15 @@ __global__ __launch_bounds__(BLOCK) void ray_kernel(int64_t n, const float* __re
 // of its first ray, origins within 1% of the scene's diagonal) walks as a packet;
 // any other wave walks lane by lane.  Both give closest_hit's results.
 constexpr int kRayLds = 8;
-__global__ __launch_bounds__(256) void ray_packet_kernel(int64_t n, const float* __restrict__ rays_o,
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ray_packet_kernel(int64_t n, const float* __restrict__ rays_o,
                                                          const float* __restrict__ rays_d,
                                                          const uint32_t* __restrict__ order,
                                                          float* __restrict__ positions, int64_t* __restrict__ face_id,
                                                          float* __restrict__ depth, Tree T) {
-    __shared__ int32_t s_ref[kRayLds * 256];
-    __shared__ float s_key[kRayLds * 256];
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ int32_t s_ref[kRayLds * 64 * WAVES];
+    __shared__ float s_key[kRayLds * 64 * WAVES];
+    const int64_t g = (int64_t)blockIdx.x * (64 * WAVES) + threadIdx.x;
     int64_t i = g;  // ray index (lane g takes ray order[g] when an order is given)
     if (g < n && order) i = (int64_t)order[g];
     const bool valid = g < n && i < n;
@@ -1253,7 +1257,7 @@ __global__ __launch_bounds__(256) void ray_packet_kernel(int64_t n, const float*
         closest_hit_packet(T, o, d, valid, t, slot, bid);
     } else {
         if (!valid) return;
-        LaneStack<256, kRayLds> st;
+        LaneStack<64 * WAVES, kRayLds> st;
         st.ref = s_ref;
         st.key = s_key;
         st.tid = (int)threadIdx.x;
@@ -1273,9 +1277,10 @@ namespace {
 
 constexpr int kBlock = 128;
 constexpr int kLdsEntries = 12;
-// Waves per workgroup of the packet distance kernels: with one, a workgroup ends with its
-// wave and the next starts at once (the 256^3 UDF walk 11.68 ms against 11.90 with 4 and
-// 13.47 with 8; the watertight SDF 889 against 868 Mq/s).
+// Waves per workgroup of the packet distance and ray kernels: with one, a workgroup ends
+// with its wave and the next starts at once (the 256^3 UDF walk 11.68 ms against 11.90
+// with 4 and 13.47 with 8; the watertight SDF 889 against 868 Mq/s; camera rays 3119
+// against 2899 Mrays/s).
 constexpr int kDistanceWaves = 1;
 
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
@@ -1377,14 +1382,15 @@ hipError_t launch_stab_sign(const QueryArgs& q, void* ws) {
     int dev_id = 0, n_cu = 256;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev_id);
-    // persistent waves: as many workgroups as the CUs hold at once (12 KB of LDS each)
-    const unsigned grid = (unsigned)std::min<int64_t>((q.n + kBlock - 1) / kBlock, (int64_t)n_cu * dev::kStabBlocksPerCu);
+    // persistent waves: as many workgroups as the CUs hold at once
+    constexpr int SB = dev::kStabBlock;
+    const unsigned grid = (unsigned)std::min<int64_t>((q.n + SB - 1) / SB, (int64_t)n_cu * dev::kStabBlocksPerCu);
     const unsigned keep_grid = (unsigned)((q.n + 255) / 256);
     hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts, q.n);
-    hipLaunchKernelGGL((dev::stab_first<kBlock, kLdsEntries>), dim3((unsigned)((q.n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       s, q.order, lists[0], counts, q.in0, offs, T);
+    hipLaunchKernelGGL((dev::stab_first<SB, kLdsEntries>), dim3((unsigned)((q.n + SB - 1) / SB)), dim3(SB), 0, s, q.order,
+                       lists[0], counts, q.in0, offs, T);
     for (int b = 0; b < dev::kStabBatches; ++b) {
-        hipLaunchKernelGGL((dev::stab_lines<kBlock, kLdsEntries>), dim3(grid), dim3(kBlock), 0, s, (uint32_t)b, lists[b & 1],
+        hipLaunchKernelGGL((dev::stab_lines<SB, kLdsEntries>), dim3(grid), dim3(SB), 0, s, (uint32_t)b, lists[b & 1],
                            counts, q.in0, offs, T);
         hipLaunchKernelGGL(dev::stab_keep, dim3(keep_grid), dim3(256), 0, s, lists[b & 1], counts + 1 + b,
                            lists[(b + 1) & 1], counts + 2 + b);
@@ -1399,9 +1405,9 @@ hipError_t launch_ray(const QueryArgs& q) {
     if (stack_capacity_for(q.stack_need) < 0) return hipErrorInvalidValue;
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     if (q.packet) {
-        const unsigned grid = (unsigned)((q.n + 255) / 256);
-        hipLaunchKernelGGL(dev::ray_packet_kernel, dim3(grid), dim3(256), 0, (hipStream_t)q.stream, q.n, q.in0, q.in1,
-                           q.order, q.out_v, q.out_id, q.out_f, T);
+        constexpr int W = kDistanceWaves;  // one wave per workgroup, as the distance kernels (3119 vs 2899 Mrays/s with 4)
+        hipLaunchKernelGGL(dev::ray_packet_kernel<W>, dim3((unsigned)((q.n + 64 * W - 1) / (64 * W))), dim3(64 * W), 0,
+                           (hipStream_t)q.stream, q.n, q.in0, q.in1, q.order, q.out_v, q.out_id, q.out_f, T);
         return hipGetLastError();
     }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
