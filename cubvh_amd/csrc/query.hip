@@ -20,6 +20,7 @@
 
 
 
+
 #include "bvh.hpp"
 #include "geom.hpp"
 #include "query.hpp"
@@ -1060,18 +1061,33 @@ __global__ __launch_bounds__(256) void stab_negate(const uint32_t* __restrict__ 
     distances[q] = -distances[q];
 }
 
+// Workgroup b of a launch runs on XCD b % 8: the dispatcher deals workgroups round-robin
+// over the 8 XCDs, each with its own L2.  xcd_remap gives workgroup b the logical block
+// it works on so that each XCD takes runs of `chunk` consecutive logical blocks (within
+// every group of 8 chunk workgroups; a last partial group keeps its order): neighbouring
+// tiles, which walk the same part of a tree, then share one L2 instead of eight.  A
+// bijection of [0, nb), so every tile is still walked once.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk) {
+    if (chunk == 0) return b;
+    const uint32_t group = 8u * chunk;
+    if (b >= nb / group * group) return b;
+    const uint32_t r = b % group;
+    return b - r + (r % 8u) * chunk + r / 8u;
+}
+
 constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal search
 
 // Packet kernel for the unsigned (MODE 0) and watertight signed (MODE 1) distance, WAVES
-// waves per workgroup.  Lane i of the grid handles query order[i] (or i without a
-// permutation).
+// waves per workgroup.  Workgroup b walks tile xcd_remap(b) (runs of xchunk consecutive
+// tiles of the Morton order per XCD); its lane i handles query order[64 tile + i] (or
+// that index without a permutation).
 template <int MODE, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 8) void distance_packet_kernel(int64_t n, const float* __restrict__ positions,
                                                               const uint32_t* __restrict__ order,
                                                               float* __restrict__ distances,
                                                               int64_t* __restrict__ face_id, float* __restrict__ uvw,
-                                                              Tree T) {
-    const int64_t i = (int64_t)blockIdx.x * (64 * WAVES) + threadIdx.x;
+                                                              Tree T, uint32_t xchunk) {
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x, xchunk) * (64 * WAVES) + threadIdx.x;
     int64_t q = i;
     if (i < n && order) q = (int64_t)order[i];
     const bool valid = i < n && q < n;  // order entries >= n are skipped
@@ -1113,20 +1129,6 @@ __global__ __launch_bounds__(64 * WAVES, 8) void distance_packet_kernel(int64_t 
     distances[q] = dist;
     face_id[q] = (int64_t)id;
     if (uvw) store_pt(uvw, q, tri_barycentric(a, b, c, cp));
-}
-
-// Workgroup b of a launch runs on XCD b % 8: the dispatcher deals workgroups round-robin
-// over the 8 XCDs, each with its own L2.  xcd_remap gives workgroup b the logical block
-// it works on so that each XCD takes runs of `chunk` consecutive logical blocks (within
-// every group of 8 chunk workgroups; a last partial group keeps its order): neighbouring
-// tiles, which walk the same part of a tree, then share one L2 instead of eight.  A
-// bijection of [0, nb), so every tile is still walked once.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk) {
-    if (chunk == 0) return b;
-    const uint32_t group = 8u * chunk;
-    if (b >= nb / group * group) return b;
-    const uint32_t r = b % group;
-    return b - r + (r % 8u) * chunk + r / 8u;
 }
 
 // Unsigned distance over a forest: one wave per 64-slot tile of the padded query
@@ -1283,18 +1285,29 @@ constexpr int kLdsEntries = 12;
 // against 2899 Mrays/s).
 constexpr int kDistanceWaves = 1;
 
+// Runs of consecutive workgroups per XCD for a grid of nb one-wave workgroups over a
+// Morton order: the largest power of two <= nb / 128, so that each XCD's L2 serves one
+// aligned block of the order at a time (the 256^3 UDF walk: 10.80 ms with 2048 =
+// 2^24 / 64 / 128, 11.38 ms without, 11.6 ms with 1536 or 3072, 11.68 ms with 8192).
+uint32_t xcd_chunk_for(unsigned nb) {
+    uint32_t c = 1;
+    while ((uint64_t)c * 2 * 128 <= nb) c *= 2;
+    return nb >= 128 * 8 ? c : 0u;
+}
+
 hipError_t launch_distance_impl(const QueryArgs& q, int mode) {
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     hipStream_t s = (hipStream_t)q.stream;
     if ((mode == 0 || mode == 1) && q.packet) {
         constexpr int W = kDistanceWaves;
         const unsigned grid = (unsigned)((q.n + 64 * W - 1) / (64 * W));
+        const uint32_t xchunk = xcd_chunk_for(grid);
         if (mode == 0)
             hipLaunchKernelGGL((dev::distance_packet_kernel<0, W>), dim3(grid), dim3(64 * W), 0, s, q.n, q.in0, q.order,
-                               q.out_f, q.out_id, q.out_v, T);
+                               q.out_f, q.out_id, q.out_v, T, xchunk);
         else
             hipLaunchKernelGGL((dev::distance_packet_kernel<1, W>), dim3(grid), dim3(64 * W), 0, s, q.n, q.in0, q.order,
-                               q.out_f, q.out_id, q.out_v, T);
+                               q.out_f, q.out_id, q.out_v, T, xchunk);
         return hipGetLastError();
     }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
@@ -1406,8 +1419,11 @@ hipError_t launch_ray(const QueryArgs& q) {
     const dev::Tree T = dev::make_tree(q.tree, q.face0_slot);
     if (q.packet) {
         constexpr int W = kDistanceWaves;  // one wave per workgroup, as the distance kernels (3119 vs 2899 Mrays/s with 4)
-        hipLaunchKernelGGL(dev::ray_packet_kernel<W>, dim3((unsigned)((q.n + 64 * W - 1) / (64 * W))), dim3(64 * W), 0,
-                           (hipStream_t)q.stream, q.n, q.in0, q.in1, q.order, q.out_v, q.out_id, q.out_f, T);
+        // no runs per XCD here: ray costs differ between image regions, and runs put the
+        // costly ones on one XCD (camera rays 2475 Mrays/s with the distance kernels' runs)
+        const unsigned grid = (unsigned)((q.n + 64 * W - 1) / (64 * W));
+        hipLaunchKernelGGL(dev::ray_packet_kernel<W>, dim3(grid), dim3(64 * W), 0, (hipStream_t)q.stream, q.n, q.in0,
+                           q.in1, q.order, q.out_v, q.out_id, q.out_f, T);
         return hipGetLastError();
     }
     const unsigned grid = (unsigned)((q.n + kBlock - 1) / kBlock);
