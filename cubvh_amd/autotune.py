@@ -29,7 +29,7 @@ CANDIDATES = [
     for persistent, wpc in ((0, 8), (1, 4), (1, 8), (1, 16))
 ]
 WAVES_PER_BLOCK = (1, 2, 4, 8)
-XCD_CHUNKS = (0, 4, 16, 64)
+XCD_CHUNKS = (0, 4, 16, 64, 256, 1024, 4096)
 MAX_BINS = 1 << 30  # n_meshes << bits (the library refuses more Morton bins)
 
 

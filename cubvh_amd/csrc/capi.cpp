@@ -1202,7 +1202,7 @@ int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, 
     const int32_t wpb = n_tuning > 3 ? tuning[3] : 4;
     const int32_t xcd = n_tuning > 4 ? tuning[4] : 0;
     if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(CUBVH_ERR_ARG, "waves per block must be 1, 2, 4 or 8");
-    if (xcd < 0 || xcd > 1024) return fail(CUBVH_ERR_ARG, "XCD chunk must be in 0..1024 workgroups");
+    if (xcd < 0 || xcd > 65536) return fail(CUBVH_ERR_ARG, "XCD chunk must be in 0..65536 workgroups");
     if (bits < 0 || bits > 16) return fail(CUBVH_ERR_ARG, "order bits must be in 0..16");
     // a bin index is mesh << bits | code in 32 bits; the histogram holds one word per bin
     if (((int64_t)forest->n_meshes << bits) > kMaxForestBins)

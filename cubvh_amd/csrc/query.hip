@@ -868,6 +868,20 @@ __global__ __launch_bounds__(BLOCK) void distance_kernel(int64_t n, const float*
     }
 }
 
+// Workgroup b of a launch runs on XCD b % 8: the dispatcher deals workgroups round-robin
+// over the 8 XCDs, each with its own L2.  xcd_remap gives workgroup b the logical block
+// it works on so that each XCD takes runs of `chunk` consecutive logical blocks (within
+// every group of 8 chunk workgroups; a last partial group keeps its order): neighbouring
+// tiles, which walk the same part of a tree, then share one L2 instead of eight.  A
+// bijection of [0, nb), so every tile is still walked once.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk) {
+    if (chunk == 0) return b;
+    const uint32_t group = 8u * chunk;
+    if (b >= nb / group * group) return b;
+    const uint32_t r = b % group;
+    return b - r + (r % 8u) * chunk + r / 8u;
+}
+
 // Raystab sign of distances already computed (src/bvh.cu:408-427): a query is inside,
 // and its distance negated, when all 64 stab rays from its point hit a face (the 32
 // stab lines, each hit both ways).  The stab directions are keyed by the query index,
@@ -992,10 +1006,10 @@ struct LineBatchJob {  // item t: line k0 + t / m of entry t % m
 template <int BLOCK, int LDS_N>
 __global__ __launch_bounds__(BLOCK) void stab_first(const uint32_t* __restrict__ order, uint32_t* __restrict__ list,
                                                     uint32_t* __restrict__ counts, const float* __restrict__ positions,
-                                                    float2* __restrict__ offs, Tree T) {
+                                                    float2* __restrict__ offs, Tree T, uint32_t xchunk) {
     __shared__ int32_t s_ref[LDS_N * BLOCK];
     __shared__ float s_key[LDS_N * BLOCK];
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x, xchunk) * BLOCK + threadIdx.x;
     LaneStack<BLOCK, LDS_N> st;
     st.ref = s_ref;
     st.key = s_key;
@@ -1061,19 +1075,6 @@ __global__ __launch_bounds__(256) void stab_negate(const uint32_t* __restrict__ 
     distances[q] = -distances[q];
 }
 
-// Workgroup b of a launch runs on XCD b % 8: the dispatcher deals workgroups round-robin
-// over the 8 XCDs, each with its own L2.  xcd_remap gives workgroup b the logical block
-// it works on so that each XCD takes runs of `chunk` consecutive logical blocks (within
-// every group of 8 chunk workgroups; a last partial group keeps its order): neighbouring
-// tiles, which walk the same part of a tree, then share one L2 instead of eight.  A
-// bijection of [0, nb), so every tile is still walked once.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk) {
-    if (chunk == 0) return b;
-    const uint32_t group = 8u * chunk;
-    if (b >= nb / group * group) return b;
-    const uint32_t r = b % group;
-    return b - r + (r % 8u) * chunk + r / 8u;
-}
 
 constexpr int kNormalLds = 4;  // LDS entries per lane of the per-lane normal search
 
@@ -1400,8 +1401,9 @@ hipError_t launch_stab_sign(const QueryArgs& q, void* ws) {
     const unsigned grid = (unsigned)std::min<int64_t>((q.n + SB - 1) / SB, (int64_t)n_cu * dev::kStabBlocksPerCu);
     const unsigned keep_grid = (unsigned)((q.n + 255) / 256);
     hipLaunchKernelGGL(dev::stab_init, dim3(1), dim3(64), 0, s, counts, q.n);
-    hipLaunchKernelGGL((dev::stab_first<SB, kLdsEntries>), dim3((unsigned)((q.n + SB - 1) / SB)), dim3(SB), 0, s, q.order,
-                       lists[0], counts, q.in0, offs, T);
+    const unsigned first_grid = (unsigned)((q.n + SB - 1) / SB);
+    hipLaunchKernelGGL((dev::stab_first<SB, kLdsEntries>), dim3(first_grid), dim3(SB), 0, s, q.order, lists[0], counts,
+                       q.in0, offs, T, xcd_chunk_for(first_grid));
     for (int b = 0; b < dev::kStabBatches; ++b) {
         hipLaunchKernelGGL((dev::stab_lines<SB, kLdsEntries>), dim3(grid), dim3(SB), 0, s, (uint32_t)b, lists[b & 1],
                            counts, q.in0, offs, T);

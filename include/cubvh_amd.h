@@ -254,7 +254,7 @@ int cubvh_forest_reserve(cubvh_forest* forest, int device, int64_t max_queries, 
  * tiles from an atomic counter, 0 = one wave per tile, [2] waves per CU of the
  * persistent grid (1..32), [3] waves per workgroup (1, 2, 4 or 8), [4] for the
  * one-wave-per-tile grid, runs of that many consecutive workgroups given to each XCD
- * (0..1024, 0 = dispatch order; neighbouring tiles then share one L2).  Grows the workspace
+ * (0..65536, 0 = dispatch order; neighbouring tiles then share one L2).  Grows the workspace
  * when the reserved one is too small, which is refused while the stream is being
  * captured. */
 int cubvh_forest_unsigned_distance(cubvh_forest* forest, int device, int64_t n, const float* positions,
