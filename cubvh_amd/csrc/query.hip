@@ -910,10 +910,12 @@ constexpr int kStabBatches = 3;
 constexpr uint32_t kStabBatchFirst[kStabBatches + 1] = {1u, 4u, 12u, 32u};
 // idle lanes of a wave take new items once this many of the 64 are idle (or all are)
 constexpr int kRefillLanes = 16;
-// waves per SIMD the persistent stab kernels are compiled for (80 VGPRs); the stab
-// kernels run one-wave workgroups (19.3 ms per call against 19.6 with two) with 6 KB of
-// LDS each, 4 x kStabWaves of them per CU
-constexpr int kStabWaves = 6;
+// waves per SIMD the persistent stab kernels are compiled for (72 VGPRs, a few spilled:
+// 18.9 ms per call against 19.3 with 6 waves and 80 VGPRs; 8 waves do not fit the LDS
+// and the compiler falls back to 5); the stab kernels run one-wave workgroups (19.3 ms
+// against 19.6 with two) with 6 KB of LDS each, 4 x kStabWaves of them per CU (26 fit
+// the LDS; the others start as the first ones leave)
+constexpr int kStabWaves = 7;
 constexpr int kStabBlock = 64;
 constexpr int kStabBlocksPerCu = 4 * kStabWaves;
 
