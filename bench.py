@@ -6,15 +6,18 @@ query pass over all 16,777,216 grid points, inputs resident in HBM: Morton-order
 points (cubvh_morton_order), then walk them (cubvh_distance_ordered) -- the two halves
 of cubvh_unsigned_distance, whose own rate is reported as config.dropin_call_Mq_s.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras] [--no-cpu-baseline] [--headline-only]
-                 [--hashtable-only] [--smc-only]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu-baseline] [--headline-only]
+                 [--hashtable-only] [--smc-only] [--valu-probe]
 
 Prints one JSON line.  `value` = M queries/s over the timed steps (wall clock
-bracketed by device synchronizations).  `roofline` prices the dominant kernel at the
-algorithmic bytes of the reference's own BVH walk on the same workload
-(tests/golden/algo_bytes.json) divided by its average duration measured with HIP
-events on the launch stream around the walk inside the timed steps; `copy_GBs` is a
-measured 1 GiB device-to-device copy rate (bytes read + written per second).  `cpu_baseline` times the CPU restatement of the
+bracketed by device synchronizations).  `roofline` is the walk kernel's: its HBM bytes
+per launch from the committed counter summary (profiles/*_counters.md) over its
+average duration, measured with HIP events on the launch stream around the walk
+inside the timed steps, against 8 TB/s; and its VALU instructions per second against
+the issue ceiling measured in the same run (cubvh_valu_probe), the bound the walks
+sit at.  `reference_walk_equiv_GBs` prices the reference's own walk on the same
+workload (tests/golden/algo_bytes.json) over the same time; `copy_GBs` is a measured
+1 GiB device-to-device copy rate.  `cpu_baseline` times the CPU restatement of the
 reference's BVH walk (oracle/) on a seeded sample of the same grid on host cores.
 The JSON also carries the metric's two other forms: `fwd_bwd` (jittered level-6
 sphere, 2^20 points, loss = mean(udf^2), forward + hand-written backward) and
@@ -22,8 +25,9 @@ sphere, 2^20 points, loss = mean(udf^2), forward + hand-written backward) and
 uniform points each: one autotuned, HIP-graph-replayed forest launch against 512
 per-mesh calls), `hashtable` (sparse-voxel hash table build and search) and
 `sparse_marching_cubes` (the cut cells of a 1024^3 lattice), each with the reference's
-CPU implementation timed beside it.  --extras adds watertight SDF, ray tracing,
-raystab, floodfill and the GPU tree build.
+CPU implementation timed beside it, and `extra`: watertight SDF and floodfill on the
+headline grid, camera rays and raystab on the soup, and the GPU tree builds (skipped
+with --no-extras, or with --headline-only unless --extras is given).
 """
 from __future__ import annotations
 
@@ -574,7 +578,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--extras", action="store_true")
+    ap.add_argument("--extras", action="store_true", help="run the extra legs also with --headline-only")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the extra legs (watertight SDF, floodfill, rays, raystab, GPU builds)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--headline-only", action="store_true")
@@ -702,7 +708,7 @@ def main():
         result["hashtable"] = hashtable_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
         result["sparse_marching_cubes"] = smc_leg(dev, max(3, args.steps), 1, with_cpu=not args.no_cpu_baseline)
     extra = {}
-    if args.extras:
+    if args.extras or not (args.no_extras or args.headline_only):
         run_extras(dev, max(2, args.steps // 2), 1, extra, gpu_build=not args.no_gpu_build)
         result["extra"] = extra
     if not args.no_cpu_baseline:
