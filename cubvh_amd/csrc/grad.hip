@@ -20,7 +20,14 @@
 //                     face's first query; a scan gives the chunks' positions, and
 //                     chunk_marks + a max-scan give each chunk its face
 //   5. chunk_sums   : one lane per chunk adds -w_k g n over its queries in order
-//   6. face_totals  : one lane per face adds its chunks' sums in order
+//   6. a sum tree of fan-in kChunk over each face's chunk sums: chunk_tree_level
+//                     passes, then face_totals for the last level.  A level adds
+//                     runs of kChunk values in order, starting from zero, so a face
+//                     with k queries costs about log_32(k) short passes instead of
+//                     k / kChunk additions on one lane; a face of at most kChunk
+//                     chunks is one run of its chunk sums, and passes beyond a face's
+//                     own depth leave its total unchanged (0 + x == x for a sum that
+//                     began at +0, never -0)
 //   7. vertex_sums  : one lane per vertex adds its corners' face totals in corner
 //                     order and adds the total to grad_vertices
 // Step 7 reads the vertex -> corners table (the corners 3 f + k of every face grouped
@@ -155,18 +162,41 @@ __global__ __launch_bounds__(256) void chunk_sums(int64_t n_faces, int64_t max_c
     for (int c = 0; c < 9; ++c) part[9 * t + c] = a[c];
 }
 
-// Lane per face: the face's chunk sums added in chunk order, written as
-// acc[3 (3 f + k) + axis] (zero for a face without queries).
-__global__ __launch_bounds__(256) void face_totals(int64_t n_faces, int64_t max_chunks,
+// One level of each face's sum tree, in place: chunk t = cbase[f] + r of face f with r
+// a multiple of kChunk * stride receives the sum, from zero and in order, of the values
+// at t + i stride (i = 0 .. kChunk - 1, inside the face's chunks).  With stride =
+// kChunk^(L-1) the values read are level L-1's and the ones written level L's; a lane
+// reads only below t + kChunk * stride, where the next writing lane of the face is.
+__global__ __launch_bounds__(256) void chunk_tree_level(int64_t n_faces, int64_t max_chunks, int64_t stride,
+                                                        const uint32_t* __restrict__ cbase,
+                                                        const uint32_t* __restrict__ cface, float* __restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = cbase[n_faces];
+    if (t >= total || t >= max_chunks) return;
+    const int64_t f = (int64_t)cface[t] - 1;
+    if ((t - (int64_t)cbase[f]) % (stride * kChunk) != 0) return;
+    const int64_t e = min((int64_t)cbase[f + 1], max_chunks);
+    float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = 0, u = t; i < kChunk && u < e; ++i, u += stride)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a[c] += part[9 * u + c];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) part[9 * t + c] = a[c];
+}
+
+// Lane per face: the last level of its sum tree (the values at cbase[f] + i stride,
+// i = 0 .. kChunk - 1, added in order from zero), written as acc[3 (3 f + k) + axis]
+// (zero for a face without queries).
+__global__ __launch_bounds__(256) void face_totals(int64_t n_faces, int64_t max_chunks, int64_t stride,
                                                    const uint32_t* __restrict__ cbase, const float* __restrict__ part,
                                                    float* __restrict__ acc) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n_faces) return;
     float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int64_t b = cbase[f], e = min((int64_t)cbase[f + 1], max_chunks);
-    for (int64_t t = b; t < e; ++t)
+    for (int64_t i = 0, u = b; i < kChunk && u < e; ++i, u += stride)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) a[c] += part[9 * t + c];
+        for (int c = 0; c < 9; ++c) a[c] += part[9 * u + c];
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[9 * f + c] = a[c];
 }
@@ -389,8 +419,15 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::chunk_sums, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc, v1, fbeg,
                        fend, cbase, cface, gn, uvw, part);
-    hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc, cbase,
-                       part, acc);
+    // levels of the faces' sum trees: a face holds at most ceil(n / kChunk) chunks, so
+    // the last level's runs of kChunk values at distance `stride` cover every face
+    const int64_t face_chunks_max = (n + dev::kChunk - 1) / dev::kChunk;
+    int64_t stride = 1;
+    for (; stride * dev::kChunk < face_chunks_max; stride *= dev::kChunk)
+        hipLaunchKernelGGL(dev::chunk_tree_level, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc,
+                           stride, cbase, cface, part);
+    hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc,
+                       stride, cbase, part, acc);
     if (!have_tables) {
         corners = (const uint32_t*)(w + L.corners);
         corner_begin = (const uint32_t*)(w + L.vbeg);

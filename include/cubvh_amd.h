@@ -186,8 +186,8 @@ int cubvh_floodfill(int device, const uint8_t* grid, int32_t B, int32_t H, int32
  * grad_points fp32 [n,3] may be NULL; grad_vertices fp32 [n_vertices,3] may be NULL and
  * is accumulated into (the caller zeroes it).  The vertex sums are gathered in a fixed
  * order without atomics (per face its queries by ascending index, summed in chunks of 32
- * that are then added in order; per vertex its corners 3 f + k ascending): the same bits
- * on every call.  With grad_vertices the call
+ * whose sums are reduced in runs of 32, each added in order, until one is left; per
+ * vertex its corners 3 f + k ascending): the same bits on every call.  With grad_vertices the call
  * allocates stream-ordered scratch (sorts of the n face ids and 3 n_faces corners);
  * n < 2^31 and 3 n_faces < 2^31. */
 int cubvh_udf_backward(int device, int64_t n, const float* grad_dist, const float* positions,

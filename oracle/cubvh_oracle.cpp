@@ -75,7 +75,8 @@
 //      envelope-theorem gradient (grad_p = g n, grad_v_k -= w_k g n with
 //      n = (p - c) / |p - c|, c rebuilt from face + uvw) in fp32 with the product's
 //      operations and summation order -- per face its queries in ascending index, in
-//      chunks of 32 summed separately and then added in order; per vertex its corners
+//      chunks of 32 summed separately, then the chunk sums reduced in runs of 32 (each
+//      run added in order from zero) until one value is left; per vertex its corners
 //      3 f + k in ascending order -- so that the product's
 //      vertex gradients can be compared bit for bit.  Pinned against FP64 central
 //      finite differences of the FP64 brute-force distance (tests/test_oracle_backward.py).
@@ -1590,13 +1591,16 @@ int orc_udf_backward(const float* V, int64_t nV, const int32_t* F, int64_t nF, c
         for (int k = 0; k < 3; ++k) gp[3 * i + k] = gn[3 * i + k];
         if (ok && s != 0.0f) by_face[(size_t)f].push_back(i);
     }
-    // per face: its queries in chunks of 32 from the first, each chunk summed in order,
-    // then the chunk sums added in order
+    // per face: its queries in chunks of 32 from the first, each chunk summed in order;
+    // then runs of 32 consecutive sums, each added in order from zero, replace the sums
+    // until one is left (a single chunk sum goes through one such run too)
     const size_t kChunk = 32;
     std::vector<float> acc(9 * (size_t)nF, 0.0f);
+    std::vector<float> sums, next;
     for (int64_t f = 0; f < nF; ++f) {
         const std::vector<int64_t>& qs = by_face[(size_t)f];
-        float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (qs.empty()) continue;
+        sums.clear();
         for (size_t c0 = 0; c0 < qs.size(); c0 += kChunk) {
             float part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (size_t j = c0; j < std::min(qs.size(), c0 + kChunk); ++j) {
@@ -1606,9 +1610,19 @@ int orc_udf_backward(const float* V, int64_t nV, const int32_t* F, int64_t nF, c
                     for (int c = 0; c < 3; ++c) part[3 * k + c] += -wk * gn[3 * q + c];
                 }
             }
-            for (int k = 0; k < 9; ++k) a[k] += part[k];
+            sums.insert(sums.end(), part, part + 9);
         }
-        for (int k = 0; k < 9; ++k) acc[9 * f + k] = a[k];
+        size_t m = sums.size() / 9;
+        do {
+            const size_t mo = (m + kChunk - 1) / kChunk;
+            next.assign(9 * mo, 0.0f);
+            for (size_t u = 0; u < mo; ++u)
+                for (size_t j = u * kChunk; j < std::min(m, u * kChunk + kChunk); ++j)
+                    for (int k = 0; k < 9; ++k) next[9 * u + k] += sums[9 * j + k];
+            sums.swap(next);
+            m = mo;
+        } while (m > 1);
+        for (int k = 0; k < 9; ++k) acc[9 * f + k] = sums[k];
     }
     std::vector<std::vector<int64_t>> by_vertex((size_t)nV);
     for (int64_t s = 0; s < 3 * nF; ++s)

@@ -111,8 +111,9 @@ def test_backward_bit_exact_vs_oracle_and_repeatable(gpu, oracle, record):
     """The differentiable-UDF workload (jittered level-6 sphere, 2^20 points N(0, 0.4^2),
     loss mean(d^2)): point and vertex gradients equal the oracle's restatement bit for bit
     (same operations, same summation order: per face its queries by ascending index in
-    chunks of 32, per vertex its corners 3 f + k ascending; this workload has faces with
-    thousands of queries), and two calls give the same bits."""
+    chunks of 32 whose sums are reduced in runs of 32, per vertex its corners 3 f + k
+    ascending; this workload has faces with thousands of queries, i.e. two levels of
+    runs), and two calls give the same bits."""
     import ctypes
 
     from cubvh_amd import _lib
@@ -214,8 +215,9 @@ def test_vertex_corner_table_and_backward_with_it(gpu, oracle):
 
 
 def test_backward_one_face_takes_almost_every_query(gpu, oracle):
-    """Skewed faces: one large triangle under 2^17 points (thousands of chunks of one
-    face) and ten small triangles far away; gradients bit-exact against the oracle."""
+    """Skewed faces: one large triangle under 2^17 points (4,096 chunks of one face,
+    reduced in three levels of runs of 32) and ten small triangles far away; gradients
+    bit-exact against the oracle."""
     import ctypes
 
     from cubvh_amd import _lib
