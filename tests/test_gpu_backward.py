@@ -246,3 +246,45 @@ def test_backward_one_face_takes_almost_every_query(gpu, oracle):
     ogp, ogv = oracle.udf_backward(v, f, pts, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
     np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
     np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
+
+
+def test_backward_face_sum_tree_levels(gpu, oracle):
+    """Faces holding exactly 1,024, 1,025 and 32,769 queries: 32, 33 and 1,025 chunks of
+    32, i.e. sum trees of one, two and three levels of runs of 32 (the call itself runs
+    three levels, its largest face could hold all 34,818 queries); gradients bit-exact
+    against the oracle."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    rng = np.random.default_rng(5)
+    big = np.array([[-2.0, -2.0, 0.0], [4.0, -2.0, 0.0], [-2.0, 4.0, 0.0]], np.float32)
+    shifts = np.array([[0.0, 0.0, 0.0], [20.0, 0.0, 0.0], [40.0, 0.0, 0.0]], np.float32)
+    far = (rng.uniform(-0.01, 0.01, size=(10, 3, 3)) + np.array([5.0, 5.0, 5.0])).reshape(-1, 3)
+    v = np.concatenate([big + s for s in shifts] + [far]).astype(np.float32)
+    f = np.arange(len(v), dtype=np.int32).reshape(-1, 3)
+    counts = (1024, 1025, 32769)
+    pts = np.concatenate([
+        np.concatenate([rng.uniform(-1, 1, size=(c, 2)), rng.uniform(0.05, 0.5, size=(c, 1))], axis=1) + shifts[k]
+        for k, c in enumerate(counts)
+    ]).astype(np.float32)
+    pts = pts[rng.permutation(len(pts))]
+    n = len(pts)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    assert [int((fid == k).sum()) for k in range(3)] == list(counts)
+    g = torch.linspace(0.25, 2.0, n, device=gpu)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(Vt)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    _lib.check(lib.cubvh_udf_backward(gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(),
+                                      Vt.data_ptr(), len(v), Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr(), s))
+    torch.cuda.synchronize()
+    ogp, ogv = oracle.udf_backward(v, f, pts, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
+    np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
+    np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
+    assert np.abs(ogv[:9, 2]).min() > 0  # every corner of the three faces takes a share
