@@ -600,10 +600,16 @@ std::string validate_bvh(HostBvh& bvh) {
         if (seen[ni]) return "node reached twice";
         seen[ni] = 1;
         max_depth = std::max(max_depth, d);
-        const WideNode& w = bvh.nodes[ni];
+        WideNode& w = bvh.nodes[ni];
         for (int k = 0; k < 4; ++k) {
             int32_t ref = w.child[k];
-            if (ref == CHILD_EMPTY) continue;
+            if (ref == CHILD_EMPTY) {
+                // the walks rely on an empty slot's box being empty (+inf .. -inf): no
+                // point with a finite or infinite coordinate reaches it
+                w.lo_x[k] = w.lo_y[k] = w.lo_z[k] = INFINITY;
+                w.hi_x[k] = w.hi_y[k] = w.hi_z[k] = -INFINITY;
+                continue;
+            }
             if (ref >= 0) {
                 st.push_back({ref, d + 1});
             } else {

@@ -89,8 +89,9 @@ std::string build_bvh(const float* tri9, const int64_t* ids, int64_t n_faces, in
 // references must already be valid (build_bvh output or validate_bvh passed).
 void compute_face_bounds(HostBvh& bvh);
 
-// Recomputes depth / stack_need / face0_slot of an imported tree and validates every
-// reference.  Returns an empty string on success.
+// Recomputes depth / stack_need / face0_slot of an imported tree, validates every
+// reference and sets the box of every empty child slot to +inf .. -inf.  Returns an
+// empty string on success.
 std::string validate_bvh(HostBvh& bvh);
 
 }  // namespace cubvh_amd

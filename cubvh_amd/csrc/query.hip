@@ -148,6 +148,12 @@ __device__ __forceinline__ void box_dist2_4(const WideNode& nd, f3 p, float d[4]
     }
 }
 
+// A point with a NaN coordinate: every squared distance to a face is NaN, so no face
+// can win (the reference's `<=` tests are false for it too) and the walks skip it.  It
+// must not walk: fmaxf drops NaN operands, so its box distances are finite, and with
+// three NaN coordinates zero even for an empty child slot (whose box is +inf .. -inf).
+__device__ __forceinline__ bool has_nan(f3 p) { return p.x != p.x || p.y != p.y || p.z != p.z; }
+
 __device__ __forceinline__ WideNode load_node(const WideNode* __restrict__ nodes, int32_t i) {
     const float4* src = reinterpret_cast<const float4*>(nodes + i);
     WideNode nd;
@@ -164,16 +170,18 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
     float best = MAX_DIST_SQ;
     float bound = MAX_DIST_SQ * BOUND_SLACK;
     int32_t bslot = -1, bid = 0x7fffffff;
-    int32_t ref = 0;
+    int32_t ref = has_nan(p) ? CHILD_EMPTY : 0;
     st.sp = 0;
-    for (;;) {
+    while (ref != CHILD_EMPTY) {
         if (ref >= 0) {
             WideNode nd = load_node(T.nodes, ref);
             float k[4];
             int32_t r[4] = {nd.child[0], nd.child[1], nd.child[2], nd.child[3]};
             box_dist2_4(nd, p, k);
-            for (int j = 0; j < 4; ++j)  // internal subtrees whose disk is out of reach
-                if (r[j] >= 0 && k[j] <= bound && disk_rejects(p, T.cbnd[8 * ref + 2 * j], T.cbnd[8 * ref + 2 * j + 1], bound))
+            for (int j = 0; j < 4; ++j)  // empty slots, and internal subtrees whose disk is out of reach
+                if (r[j] == CHILD_EMPTY ||
+                    (r[j] >= 0 && k[j] <= bound &&
+                     disk_rejects(p, T.cbnd[8 * ref + 2 * j], T.cbnd[8 * ref + 2 * j + 1], bound)))
                     k[j] = INFINITY;
             sort4(k, r);
             if (k[0] <= bound) {
@@ -209,7 +217,6 @@ __device__ int32_t closest_face(const Tree& T, f3 p, LaneStack<BLOCK, LDS_N>& st
                 break;
             }
         }
-        if (ref == CHILD_EMPTY) break;
     }
     best_d2 = best;
     best_id = bid;
@@ -408,13 +415,14 @@ __device__ f3 avg_normal(const Tree& T, f3 c, LaneStack<BLOCK, LDS_N>& st) {
     int32_t ref = 0;
     st.sp = 0;
     for (;;) {
+        if (has_nan(c)) break;  // no face is within NORMAL_EPS of it: the mean is 0 / 0
         if (ref >= 0) {
             WideNode nd = load_node(T.nodes, ref);
             float k[4];
             box_dist2_4(nd, c, k);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if (k[j] < NORMAL_EPS) st.push(nd.child[j], k[j]);
+                if (nd.child[j] != CHILD_EMPTY && k[j] < NORMAL_EPS) st.push(nd.child[j], k[j]);
         } else {
             int32_t first, count;
             leaf_range(ref, first, count);
@@ -621,7 +629,7 @@ __device__ void closest_face_packet(const Tree& T, int32_t root, f3 p, bool acti
     float bound = MAX_DIST_SQ * BOUND_SLACK;
     bslot = -1;
     bid = 0x7fffffff;
-    uint64_t mask = ballot(active);
+    uint64_t mask = ballot(active && !has_nan(p));
     if (mask == 0) return;
     WaveStack st;
     int32_t qn = 0;  // queued pairs (wave-uniform)

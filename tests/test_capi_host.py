@@ -109,6 +109,28 @@ def test_own_state_dict_validation(ico4):
         cubvh_amd.cuBVH.from_state_dict({"nodes": nodes, "triangles": sd["triangles"]})
 
 
+def test_imported_empty_slots_get_empty_boxes(ico4):
+    """An imported node whose empty child slot carries a box (finite or NaN) has that box
+    reset to +inf .. -inf: the walks never reach an empty slot through its box."""
+    import cubvh_amd
+
+    v, f = ico4
+    sd = cubvh_amd.cuBVH(v, f).state_dict()
+    nodes = sd["nodes"].clone()
+    child = nodes[:, 24:28]
+    empty = (child == -1).nonzero()
+    assert len(empty) > 0  # the tree has nodes with fewer than 4 children
+    boxes = nodes[:, :24].view(torch.float32).view(-1, 6, 4)  # lo_x, hi_x, lo_y, hi_y, lo_z, hi_z per slot
+    ref = boxes.clone()
+    for n, k in empty.tolist()[:8]:
+        boxes[n, :, k] = torch.tensor([0.0, 0.0, float("nan"), 1.0, -1.0, 1.0])
+    out = cubvh_amd.cuBVH.from_state_dict({"nodes": nodes, "triangles": sd["triangles"]}).state_dict()["nodes"]
+    got = out[:, :24].view(torch.float32).view(-1, 6, 4)
+    assert torch.equal(got, ref)
+    e = child == -1
+    assert torch.isinf(got[:, 0::2][e.unsqueeze(1).expand(-1, 3, -1)]).all()
+
+
 def test_bad_meshes_raise(ico4):
     import cubvh_amd
 

@@ -123,3 +123,22 @@ def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
     d, f, _ = forest.unsigned_distance(P, torch.from_numpy(offsets), False, best)
     rd, rf, _ = _per_mesh(bvhs, pts, offsets, gpu)
     assert torch.equal(d, rd) and torch.equal(f, rf)
+
+
+def test_forest_nan_and_inf_queries_match_per_mesh(gpu):
+    """Queries with NaN or infinite coordinates in a batch: each gets what the per-mesh
+    call gives it (the not-found answer), in the persistent and one-wave-per-tile
+    launches, and the other queries are unaffected."""
+    bvhs, pts, offsets = _forest_case(2)
+    pts[::7, 0] = np.nan
+    pts[3::11, 1] = np.inf
+    pts[5::13, 2] = -np.inf
+    forest = cubvh_amd.cuBVHForest(bvhs)
+    rd, rf, ru = (t.cpu().numpy() for t in _per_mesh(bvhs, pts, offsets, gpu))
+    bad = ~np.isfinite(pts).all(axis=1)
+    assert (rd[bad] == 0).all() and (rd[~bad] > 0).all()
+    for tuning in [(12, 1, 8), (15, 0, 8, 1, 64), (0, 0, 8)]:
+        d, f, u = forest.unsigned_distance(torch.from_numpy(pts).to(gpu), torch.from_numpy(offsets), True, tuning)
+        np.testing.assert_array_equal(d.cpu().numpy(), rd)
+        np.testing.assert_array_equal(f.cpu().numpy(), rf)
+        np.testing.assert_array_equal(u.cpu().numpy(), ru)

@@ -349,3 +349,42 @@ def test_valu_probe_counts_and_times(gpu):
             out[form, iters] = ms.value
         assert out[form, 8000] > 2.0 * out[form, 2000]
     assert out[1, 8000] > 0.9 * out[0, 8000]
+
+
+def test_nan_and_inf_queries(gpu, oracle, ico4):
+    """Query points and rays with NaN or infinite coordinates.  The reference's walks
+    compare with `<=` and find nothing for them (bvh.cu:300-353, 250-298): a point gets
+    the not-found answer (face 0, distance 0; uvw from face 0), a ray a miss; the other
+    queries of the same call are unaffected.  Bit-exact against the oracle, on the
+    ordered packet path (4,096 queries) and the per-lane path (500)."""
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    for n in (4096, 500):
+        p = S.uniform_points(n, seed=21)
+        p[::7, 0] = np.nan
+        p[3::11, 1] = np.inf
+        p[5::13, 2] = -np.inf
+        P = torch.from_numpy(p).to(gpu)
+        d, fi, uvw = bvh.unsigned_distance(P, return_uvw=True)
+        od, of, ou = oracle.udf_brute(v, f, p)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(uvw), ou)
+        bad = ~np.isfinite(p).all(axis=1)
+        assert (od[bad] == 0).all() and (of[bad] == 0).all() and (od[~bad] > 0).all()
+        for mode, m in (("watertight", 0), ("raystab", 1)):
+            d, fi, _ = bvh.signed_distance(P, mode=mode)
+            od, of, _ = oracle.sdf_brute(v, f, p, m, return_uvw=False)
+            np.testing.assert_array_equal(_np(d), od)
+            np.testing.assert_array_equal(_np(fi), of)
+        ro, rd = S.sphere_rays(n, seed=22)
+        ro[::9, 1] = np.nan
+        rd[4::10, 0] = np.nan
+        ro[2::15, 2] = np.inf
+        pos, fi, t = bvh.ray_trace(torch.from_numpy(ro).to(gpu), torch.from_numpy(rd).to(gpu))
+        opos, of, ot = oracle.ray_brute(v, f, ro, rd)
+        np.testing.assert_array_equal(_np(t), ot)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(pos), opos)
+        rbad = ~(np.isfinite(ro).all(axis=1) & np.isfinite(rd).all(axis=1))
+        assert (of[rbad] == -1).all() and (of[~rbad] >= 0).any()

@@ -71,6 +71,38 @@ def test_refbvh_rays_equal_brute_force(oracle, ico4):
     np.testing.assert_array_equal(t0, t)
 
 
+def test_nan_and_inf_queries_find_nothing(oracle, ico4):
+    """NaN or infinite coordinates: the restated reference walks (whose `<=` tests are
+    false for them, bvh.cu:300-353, 250-298) and the brute force agree on the not-found
+    answers: face 0 at distance 0 for a point, a miss at t = 1000 for a ray."""
+    from cubvh_amd import synthetic as S
+
+    v, f = ico4
+    p = S.uniform_points(500, seed=21)
+    p[::7, 0] = np.nan
+    p[3::11, 1] = np.inf
+    p[5::13, 2] = -np.inf
+    bad = ~np.isfinite(p).all(axis=1)
+    d, fi, uvw = oracle.udf_brute(v, f, p)
+    R = oracle.RefBvh(v, f)
+    wd, wf, wu = R.udf(p, tie_mode=1)[:3]
+    np.testing.assert_array_equal(wd, d)
+    np.testing.assert_array_equal(wf, fi)
+    np.testing.assert_array_equal(wu, uvw)
+    assert (d[bad] == 0).all() and (fi[bad] == 0).all() and (d[~bad] > 0).all()
+    ro, rd = S.sphere_rays(500, seed=22)
+    ro[::9, 1] = np.nan
+    rd[4::10, 0] = np.nan
+    ro[2::15, 2] = np.inf
+    rbad = ~(np.isfinite(ro).all(axis=1) & np.isfinite(rd).all(axis=1))
+    pos, fr, t = oracle.ray_brute(v, f, ro, rd)
+    pc, fc, tc, _ = R.ray(ro, rd, tie_mode=1)
+    np.testing.assert_array_equal(tc, t)
+    np.testing.assert_array_equal(fc, fr)
+    np.testing.assert_array_equal(pc, pos)
+    assert (fr[rbad] == -1).all() and (t[rbad] == 1000.0).all() and (fr[~rbad] >= 0).any()
+
+
 def test_sdf_signs(oracle, ico4):
     """Watertight and raystab signs agree with the FP64 winding number away from the
     surface; the reference-BVH restatement agrees with the brute force."""
