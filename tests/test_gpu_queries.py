@@ -388,3 +388,57 @@ def test_nan_and_inf_queries(gpu, oracle, ico4):
         np.testing.assert_array_equal(_np(pos), opos)
         rbad = ~(np.isfinite(ro).all(axis=1) & np.isfinite(rd).all(axis=1))
         assert (of[rbad] == -1).all() and (of[~rbad] >= 0).any()
+
+
+def _degenerate_mesh(v, f, seed=4):
+    """ico4 with 40 faces collapsed to segments (two corners on one vertex), 20 to points
+    and 30 turned into collinear slivers (a third vertex on the segment of the other two):
+    zero-length edges and zero normals, whose 0 / 0 terms the reference formula carries
+    through its clamp and min (triangle.cuh:46-68, common.h:53-56)."""
+    v = v.astype(np.float32)
+    f = f.astype(np.int32).copy()
+    idx = np.random.default_rng(seed).permutation(len(f))
+    f[idx[:40], 1] = f[idx[:40], 0]
+    f[idx[40:60], 1] = f[idx[40:60], 0]
+    f[idx[40:60], 2] = f[idx[40:60], 0]
+    k = idx[60:90]
+    mid = (0.25 * v[f[k, 0]] + 0.75 * v[f[k, 1]]).astype(np.float32)
+    f[k, 2] = len(v) + np.arange(len(k))
+    deg = np.zeros(len(f), bool)
+    deg[idx[:90]] = True
+    return np.concatenate([v, mid]).astype(np.float32), f, deg
+
+
+def test_degenerate_faces_bit_exact(gpu, oracle, ico4):
+    """Meshes with zero-area faces: distances, faces and uvw (some queries are won by a
+    degenerate face), both signed-distance modes and rays equal the oracle bit for bit on
+    the host-built tree, on the ordered packet path and the per-lane path; the GPU-built
+    trees give the host tree's answers."""
+    v, f, deg = _degenerate_mesh(*ico4)
+    bvh = cubvh_amd.cuBVH(v, f)
+    p = S.uniform_points(20000, seed=5)
+    for q in (p, p[:1000]):
+        d, fi, uvw = bvh.unsigned_distance(torch.from_numpy(q).to(gpu), return_uvw=True)
+        od, of, ou = oracle.udf_brute(v, f, q)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(uvw), ou)
+    assert deg[of].any()
+    P = torch.from_numpy(p).to(gpu)
+    for mode, m in (("watertight", 0), ("raystab", 1)):
+        d, fi, uvw = bvh.signed_distance(P, return_uvw=True, mode=mode)
+        od, of, ou = oracle.sdf_brute(v, f, p, m)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(uvw), ou)
+    ro, rd = S.sphere_rays(20000, seed=6)
+    pos, fi, t = bvh.ray_trace(torch.from_numpy(ro).to(gpu), torch.from_numpy(rd).to(gpu))
+    opos, of, ot = oracle.ray_brute(v, f, ro, rd)
+    np.testing.assert_array_equal(_np(t), ot)
+    np.testing.assert_array_equal(_np(fi), of)
+    np.testing.assert_array_equal(_np(pos), opos)
+    ref = [_np(x) for x in bvh.unsigned_distance(P, return_uvw=True)]
+    for build in ("gpu", "gpu_sah", "gpu_lbvh"):
+        g = cubvh_amd.cuBVH(v, f, build=build)
+        for x, y in zip(g.unsigned_distance(P, return_uvw=True), ref):
+            np.testing.assert_array_equal(_np(x), y)
