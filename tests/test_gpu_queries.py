@@ -442,3 +442,33 @@ def test_degenerate_faces_bit_exact(gpu, oracle, ico4):
         g = cubvh_amd.cuBVH(v, f, build=build)
         for x, y in zip(g.unsigned_distance(P, return_uvw=True), ref):
             np.testing.assert_array_equal(_np(x), y)
+
+
+def test_ray_edge_cases_bit_exact(gpu, oracle, ico4):
+    """Rays with a zero direction, axis-parallel directions (zero components), origins on
+    a vertex and on a face's centroid, and directions in a face's plane: hits, faces and
+    depths equal the oracle's, on the ordered packet path (4,096 rays) and the per-lane
+    path (500)."""
+    v, f = ico4
+    bvh = cubvh_amd.cuBVH(v, f)
+    rng = np.random.default_rng(31)
+    ro, rd = S.sphere_rays(4096, seed=32)
+    ro, rd = ro.copy(), rd.copy()
+    rd[0:64] = 0.0  # zero direction
+    axes = np.eye(3, dtype=np.float32)
+    rd[64:256] = np.concatenate([axes, -axes])[rng.integers(0, 6, 192)]
+    vi = rng.integers(0, len(v), 256)
+    ro[256:512] = v[vi]  # origins on vertices
+    tri = v[f[rng.integers(0, len(f), 256)]]
+    ro[512:768] = tri.mean(axis=1)  # origins on face centroids
+    e = tri[:, 1] - tri[:, 0]
+    rd[768:1024] = e / np.linalg.norm(e, axis=1, keepdims=True)  # in the face's plane
+    ro[768:1024] = tri[:, 0] - 0.5 * rd[768:1024]
+    ro, rd = ro.astype(np.float32), rd.astype(np.float32)
+    for n in (4096, 500):
+        pos, fi, t = bvh.ray_trace(torch.from_numpy(ro[:n]).to(gpu), torch.from_numpy(rd[:n]).to(gpu))
+        opos, of, ot = oracle.ray_brute(v, f, ro[:n], rd[:n])
+        np.testing.assert_array_equal(_np(t), ot)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(pos), opos)
+    assert (of[:64] == -1).all()
