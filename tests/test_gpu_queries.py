@@ -472,3 +472,32 @@ def test_ray_edge_cases_bit_exact(gpu, oracle, ico4):
         np.testing.assert_array_equal(_np(fi), of)
         np.testing.assert_array_equal(_np(pos), opos)
     assert (of[:64] == -1).all()
+
+
+def test_points_on_the_surface_bit_exact(gpu, oracle, ico4):
+    """Query points on vertices, edge midpoints and face centroids (distance 0 or within
+    rounding of it, faces tied at 0, a search bound of 0): distances, faces (the lowest
+    tied index), uvw and both signed-distance modes equal the oracle's brute force, on
+    the ordered packet path (4,096 points) and the per-lane path (500)."""
+    v, f = ico4
+    rng = np.random.default_rng(41)
+    tri = v[f[rng.integers(0, len(f), 1000)]]
+    p = np.concatenate([v[rng.integers(0, len(v), 1000)], 0.5 * (tri[:, 0] + tri[:, 1]), tri.mean(axis=1)])
+    p = np.concatenate([p, S.uniform_points(4096 - len(p), seed=42)]).astype(np.float32)
+    p = p[rng.permutation(len(p))]
+    bvh = cubvh_amd.cuBVH(v, f)
+    for q in (p, p[:500]):
+        Q = torch.from_numpy(q).to(gpu)
+        d, fi, uvw = bvh.unsigned_distance(Q, return_uvw=True)
+        od, of, ou = oracle.udf_brute(v, f, q)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(uvw), ou)
+        for mode, m in (("watertight", 0), ("raystab", 1)):
+            d, fi, uvw = bvh.signed_distance(Q, return_uvw=True, mode=mode)
+            od, of, ou = oracle.sdf_brute(v, f, q, m)
+            np.testing.assert_array_equal(_np(d), od)
+            np.testing.assert_array_equal(np.signbit(_np(d)), np.signbit(od))
+            np.testing.assert_array_equal(_np(fi), of)
+            np.testing.assert_array_equal(_np(uvw), ou)
+    assert (oracle.udf_brute(v, f, p)[0] == 0).sum() > 500
