@@ -288,3 +288,45 @@ def test_backward_face_sum_tree_levels(gpu, oracle):
     np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
     np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
     assert np.abs(ogv[:9, 2]).min() > 0  # every corner of the three faces takes a share
+
+
+def test_backward_edge_inputs_bit_exact(gpu, oracle):
+    """Backward on the outputs of edge inputs: points on vertices and face centroids
+    (distance 0: the zero subgradient), points with NaN or infinite coordinates (face 0,
+    NaN uvw: NaN point gradients, nothing added to the vertices) and a mesh with
+    zero-area faces; point and vertex gradients equal the oracle's bit for bit."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    v, f = S.jittered_sphere(3, seed=2)
+    v = v.astype(np.float32)
+    f = f.astype(np.int32).copy()
+    rng = np.random.default_rng(51)
+    idx = rng.permutation(len(f))
+    f[idx[:20], 1] = f[idx[:20], 0]  # segments
+    f[idx[20:30], 1:] = f[idx[20:30], :1]  # points
+    tri = v[f[rng.integers(0, len(f), 300)]]
+    p = np.concatenate([v[rng.integers(0, len(v), 300)], tri.mean(axis=1), S.normal_points(3000, 0.4, seed=52)])
+    p = p.astype(np.float32)
+    p[::17, 1] = np.nan
+    p[5::23, 0] = np.inf
+    p = p[rng.permutation(len(p))]
+    n = len(p)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(p).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    g = torch.linspace(0.5, 1.5, n, device=gpu)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(Vt)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    _lib.check(lib.cubvh_udf_backward(gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(),
+                                      Vt.data_ptr(), len(v), Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr(), s))
+    torch.cuda.synchronize()
+    ogp, ogv = oracle.udf_backward(v, f, p, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
+    np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
+    np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
+    assert np.isfinite(ogv).all() and np.isnan(ogp).any() and (ogp == 0).all(axis=1).any()
