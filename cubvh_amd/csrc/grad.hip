@@ -214,16 +214,19 @@ __global__ __launch_bounds__(256) void corner_keys(int64_t n_corners, const int3
 }
 
 // Lane per vertex: adds the sums of its corners corners[begin[v] .. end[v]) (corner
-// index ascending) to gv.
-__global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, const uint32_t* __restrict__ corners,
+// index ascending) to gv.  A table passed in by the caller is device data the host does
+// not check: ranges and corner indices outside the n_corners corners are skipped.
+__global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, int64_t n_corners,
+                                                   const uint32_t* __restrict__ corners,
                                                    const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
                                                    const float* __restrict__ acc, float* __restrict__ gv) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_vertices) return;
-    const int64_t lo = begin[v], hi = end[v];
+    const int64_t lo = begin[v], hi = min((int64_t)end[v], n_corners);
     float x = 0.0f, y = 0.0f, z = 0.0f;
     for (int64_t j = lo; j < hi; ++j) {
         const int64_t s = corners[j];
+        if (s >= n_corners) continue;
         x += acc[3 * s];
         y += acc[3 * s + 1];
         z += acc[3 * s + 2];
@@ -437,7 +440,7 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(dev::vertex_sums, dim3((unsigned)((n_vertices + 255) / 256)), dim3(256), 0, stream, n_vertices,
-                       corners, corner_begin, corner_end, acc, grad_vertices);
+                       3 * n_faces, corners, corner_begin, corner_end, acc, grad_vertices);
     return hipGetLastError();
 }
 

@@ -207,7 +207,9 @@ int cubvh_vertex_corners(int device, const int32_t* faces, int64_t n_faces, int6
 /* cubvh_udf_backward with the mesh's vertex -> corners table built beforehand by
  * cubvh_vertex_corners from the same faces (for callers that run many backward passes
  * over one mesh: each call then sorts only its n face ids).  The same bits as
- * cubvh_udf_backward. */
+ * cubvh_udf_backward.  The table is device data and is not checked beyond this: table
+ * entries outside the 3 n_faces corners are skipped, so a table of other faces gives
+ * wrong vertex gradients but reads no memory outside it. */
 int cubvh_udf_backward_with_corners(int device, int64_t n, const float* grad_dist, const float* positions,
                                     const int64_t* face_id, const float* uvw, const float* vertices,
                                     int64_t n_vertices, const int32_t* faces, int64_t n_faces, float* grad_points,

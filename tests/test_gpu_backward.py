@@ -330,3 +330,39 @@ def test_backward_edge_inputs_bit_exact(gpu, oracle):
     np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
     np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
     assert np.isfinite(ogv).all() and np.isnan(ogp).any() and (ogp == 0).all(axis=1).any()
+
+
+def test_backward_with_out_of_range_corner_table_entries(gpu):
+    """A caller's vertex -> corners table whose ranges run past the mesh's corners and
+    whose entries are all out of range: vertex_sums skips them (no vertex gradient, no
+    read outside the table), and the point gradients are those of the plain call."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    v, f = S.jittered_sphere(2, seed=3)
+    pts = S.normal_points(4096, 0.4, seed=4)
+    n = len(pts)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(pts).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    g = torch.ones(n, device=gpu)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    corners = torch.full((3 * len(f),), -1, dtype=torch.int32, device=gpu)  # 0xffffffff
+    begin = torch.zeros(len(v), dtype=torch.int32, device=gpu)
+    end = torch.full((len(v),), 2**31 - 1, dtype=torch.int32, device=gpu)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(Vt)
+    _lib.check(lib.cubvh_udf_backward_with_corners(
+        gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(), Vt.data_ptr(), len(v),
+        Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr(), corners.data_ptr(), begin.data_ptr(), end.data_ptr(), s))
+    gp0 = torch.empty_like(P)
+    gv0 = torch.zeros_like(Vt)
+    _lib.check(lib.cubvh_udf_backward(gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(),
+                                      Vt.data_ptr(), len(v), Ft.data_ptr(), len(f), gp0.data_ptr(), gv0.data_ptr(), s))
+    torch.cuda.synchronize()
+    assert torch.equal(gp, gp0)
+    assert (gv == 0).all() and (gv0 != 0).any()
