@@ -20,16 +20,19 @@
 //                     face's first query; a scan gives the chunks' positions, and
 //                     chunk_marks + a max-scan give each chunk its face
 //   5. chunk_sums   : one lane per chunk adds -w_k g n over its queries in order
-//   6. a sum tree of fan-in kChunk over each face's chunk sums: chunk_tree_level
-//                     passes, then face_totals for the last level.  A level adds
-//                     runs of kChunk values in order, starting from zero, so a face
-//                     with k queries costs about log_32(k) short passes instead of
-//                     k / kChunk additions on one lane; a face of at most kChunk
-//                     chunks is one run of its chunk sums, and passes beyond a face's
-//                     own depth leave its total unchanged (0 + x == x for a sum that
-//                     began at +0, never -0)
+//   6. a sum tree of fan-in kChunk over each face's chunk sums (chunk_tree_level
+//                     passes, in place): a level adds runs of kChunk values in order,
+//                     starting from zero, and leaves each run's sum at the run's first
+//                     chunk, so a face with k queries costs about log_32(k) short passes
+//                     instead of k / kChunk additions on one lane, and its total ends at
+//                     its first chunk.  A face of at most kChunk chunks is one run of
+//                     its chunk sums; passes beyond a face's own depth leave its total
+//                     unchanged (0 + x == x for a sum that began at +0, never -0), so a
+//                     pass runs only when chunk_marks found a face with more chunks
+//                     than its stride
 //   7. vertex_sums  : one lane per vertex adds its corners' face totals in corner
-//                     order and adds the total to grad_vertices
+//                     order (skipping faces without queries, whose +0 would change
+//                     nothing) and adds the total to grad_vertices
 // Step 7 reads the vertex -> corners table (the corners 3 f + k of every face grouped
 // by vertex, ascending inside a vertex: a stable radix sort of the corners by vertex
 // and run_bounds over it).  It depends only on the faces, so a caller that runs many
@@ -93,6 +96,9 @@ __global__ __launch_bounds__(256) void query_terms(int64_t n, const float* __res
 }
 
 constexpr int kChunk = 32;  // queries per chunk of a face's sorted queries
+// Passes of the faces' sum trees: strides kChunk^0 .. kChunk^(kTreeLevels - 1) cover
+// every face of a call below 2^31 queries (at most 2^26 chunks per face).
+constexpr int kTreeLevels = 6;
 
 __global__ __launch_bounds__(256) void grad_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,13 +129,19 @@ __global__ __launch_bounds__(256) void face_chunks(int64_t n_faces, const uint32
 
 // Per face with queries: marks its first chunk with f + 1 (mark holds zeros elsewhere);
 // an inclusive max-scan of the marks then gives every chunk its face + 1, whatever the
-// number of chunks of one face.
+// number of chunks of one face.  need[j] (zeroed beforehand) is set when some face has
+// more than kChunk^j chunks: sum-tree pass j changes nothing otherwise.
 __global__ __launch_bounds__(256) void chunk_marks(int64_t n_faces, int64_t max_chunks,
                                                    const uint32_t* __restrict__ nchunk,
-                                                   const uint32_t* __restrict__ cbase, uint32_t* __restrict__ mark) {
+                                                   const uint32_t* __restrict__ cbase, uint32_t* __restrict__ mark,
+                                                   uint32_t* __restrict__ need) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n_faces || nchunk[f] == 0u || (int64_t)cbase[f] >= max_chunks) return;
     mark[cbase[f]] = (uint32_t)f + 1u;
+    const uint32_t c = nchunk[f];
+    uint32_t stride = 1u;
+    for (int j = 0; j < kTreeLevels && c > stride; ++j, stride *= (uint32_t)kChunk)
+        if (need[j] == 0u) need[j] = 1u;  // every writer stores the same value
 }
 
 // Lane per chunk t < cbase[n_faces]: chunk k = t - cbase[f] of face f = cface[t] - 1 covers
@@ -165,40 +177,26 @@ __global__ __launch_bounds__(256) void chunk_sums(int64_t n_faces, int64_t max_c
 // One level of each face's sum tree, in place: chunk t = cbase[f] + r of face f with r
 // a multiple of kChunk * stride receives the sum, from zero and in order, of the values
 // at t + i stride (i = 0 .. kChunk - 1, inside the face's chunks).  With stride =
-// kChunk^(L-1) the values read are level L-1's and the ones written level L's; a lane
-// reads only below t + kChunk * stride, where the next writing lane of the face is.
+// kChunk^(L-1) the values read are level L-1's and the ones written level L's; a chunk
+// reads only below t + kChunk * stride, where the face's next written chunk is.  The
+// chunks are taken grid-stride, so that a pass with nothing to do is one small launch.
 __global__ __launch_bounds__(256) void chunk_tree_level(int64_t n_faces, int64_t max_chunks, int64_t stride,
+                                                        const uint32_t* __restrict__ need_level,
                                                         const uint32_t* __restrict__ cbase,
                                                         const uint32_t* __restrict__ cface, float* __restrict__ part) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = cbase[n_faces];
-    if (t >= total || t >= max_chunks) return;
-    const int64_t f = (int64_t)cface[t] - 1;
-    if ((t - (int64_t)cbase[f]) % (stride * kChunk) != 0) return;
-    const int64_t e = min((int64_t)cbase[f + 1], max_chunks);
-    float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t i = 0, u = t; i < kChunk && u < e; ++i, u += stride)
+    if (*need_level == 0u) return;  // no face has more chunks than `stride`
+    const int64_t total = min((int64_t)cbase[n_faces], max_chunks);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = (int64_t)cface[t] - 1;
+        if ((t - (int64_t)cbase[f]) % (stride * kChunk) != 0) continue;
+        const int64_t e = min((int64_t)cbase[f + 1], max_chunks);
+        float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t i = 0, u = t; i < kChunk && u < e; ++i, u += stride)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) a[c] += part[9 * u + c];
+            for (int c = 0; c < 9; ++c) a[c] += part[9 * u + c];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) part[9 * t + c] = a[c];
-}
-
-// Lane per face: the last level of its sum tree (the values at cbase[f] + i stride,
-// i = 0 .. kChunk - 1, added in order from zero), written as acc[3 (3 f + k) + axis]
-// (zero for a face without queries).
-__global__ __launch_bounds__(256) void face_totals(int64_t n_faces, int64_t max_chunks, int64_t stride,
-                                                   const uint32_t* __restrict__ cbase, const float* __restrict__ part,
-                                                   float* __restrict__ acc) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n_faces) return;
-    float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int64_t b = cbase[f], e = min((int64_t)cbase[f + 1], max_chunks);
-    for (int64_t i = 0, u = b; i < kChunk && u < e; ++i, u += stride)
-#pragma unroll
-        for (int c = 0; c < 9; ++c) a[c] += part[9 * u + c];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) acc[9 * f + c] = a[c];
+        for (int c = 0; c < 9; ++c) part[9 * t + c] = a[c];
+    }
 }
 
 // Corner keys: corner s = 3 f + k of every face has key F[s] (its vertex), or
@@ -213,13 +211,16 @@ __global__ __launch_bounds__(256) void corner_keys(int64_t n_corners, const int3
     val[s] = (uint32_t)s;
 }
 
-// Lane per vertex: adds the sums of its corners corners[begin[v] .. end[v]) (corner
-// index ascending) to gv.  A table passed in by the caller is device data the host does
-// not check: ranges and corner indices outside the n_corners corners are skipped.
-__global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, int64_t n_corners,
+// Lane per vertex: adds the totals of its corners corners[begin[v] .. end[v]) (corner
+// index ascending) to gv; the total of corner s = 3 f + k is part[9 cbase[f] + 3 k ..],
+// and a face without queries adds nothing.  A table passed in by the caller is device
+// data the host does not check: ranges and corner indices outside the n_corners corners
+// are skipped.
+__global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, int64_t n_corners, int64_t max_chunks,
                                                    const uint32_t* __restrict__ corners,
                                                    const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
-                                                   const float* __restrict__ acc, float* __restrict__ gv) {
+                                                   const uint32_t* __restrict__ cbase, const float* __restrict__ part,
+                                                   float* __restrict__ gv) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_vertices) return;
     const int64_t lo = begin[v], hi = min((int64_t)end[v], n_corners);
@@ -227,9 +228,13 @@ __global__ __launch_bounds__(256) void vertex_sums(int64_t n_vertices, int64_t n
     for (int64_t j = lo; j < hi; ++j) {
         const int64_t s = corners[j];
         if (s >= n_corners) continue;
-        x += acc[3 * s];
-        y += acc[3 * s + 1];
-        z += acc[3 * s + 2];
+        const int64_t f = s / 3;
+        const int64_t c = cbase[f];
+        if (c == (int64_t)cbase[f + 1] || c >= max_chunks) continue;
+        const float* a = part + 9 * c + 3 * (s - 3 * f);
+        x += a[0];
+        y += a[1];
+        z += a[2];
     }
     gv[3 * v] += x;
     gv[3 * v + 1] += y;
@@ -297,7 +302,7 @@ CornerLayout corner_layout(int64_t n_vertices, int64_t n_faces) {
 }
 
 struct BwdLayout {
-    size_t gn = 0, k0 = 0, v0 = 0, k1 = 0, v1 = 0, acc = 0, tmp = 0, total = 0;
+    size_t gn = 0, k0 = 0, v0 = 0, k1 = 0, v1 = 0, need = 0, tmp = 0, total = 0;
     size_t fbeg = 0, fend = 0, nchunk = 0, cbase = 0, mark = 0, cface = 0, part = 0;
     size_t corners = 0, vbeg = 0, vend = 0, corner_ws = 0;
     size_t tmp_bytes = 0;
@@ -316,7 +321,7 @@ BwdLayout bwd_layout(int64_t n, int64_t n_vertices, int64_t n_faces, bool need_g
     L.v0 = take(4 * (size_t)n);
     L.k1 = take(4 * (size_t)n);
     L.v1 = take(4 * (size_t)n);
-    L.acc = take(36 * (size_t)n_faces);
+    L.need = take(4 * (size_t)dev::kTreeLevels);
     L.fbeg = take(4 * (size_t)n_faces);
     L.fend = take(4 * (size_t)n_faces);
     L.nchunk = take(4 * (size_t)(n_faces + 1));
@@ -389,7 +394,6 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     float* gn = grad_points ? grad_points : (float*)(w + L.gn);
     uint32_t *k0 = (uint32_t*)(w + L.k0), *v0 = (uint32_t*)(w + L.v0);
     uint32_t *k1 = (uint32_t*)(w + L.k1), *v1 = (uint32_t*)(w + L.v1);
-    float* acc = (float*)(w + L.acc);
     void* tmp = w + L.tmp;
     hipError_t e;
     hipLaunchKernelGGL(dev::query_terms, dim3(gq), dim3(256), 0, stream, n, grad_dist, positions, face_id, uvw,
@@ -415,22 +419,22 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, nchunk, cbase, (int)(n_faces + 1), stream);
     if (e != hipSuccess) return e;
     uint32_t* mark = (uint32_t*)(w + L.mark);
+    uint32_t* need = (uint32_t*)(w + L.need);
     hipLaunchKernelGGL(dev::grad_fill_u32, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, mark, mc, 0u);
-    hipLaunchKernelGGL(dev::chunk_marks, dim3(gf), dim3(256), 0, stream, n_faces, mc, nchunk, cbase, mark);
+    hipLaunchKernelGGL(dev::grad_fill_u32, dim3(1), dim3(256), 0, stream, need, (int64_t)dev::kTreeLevels, 0u);
+    hipLaunchKernelGGL(dev::chunk_marks, dim3(gf), dim3(256), 0, stream, n_faces, mc, nchunk, cbase, mark, need);
     tb = L.tmp_bytes;
     e = hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, cface, hipcub::Max(), (int)mc, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::chunk_sums, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc, v1, fbeg,
                        fend, cbase, cface, gn, uvw, part);
-    // levels of the faces' sum trees: a face holds at most ceil(n / kChunk) chunks, so
-    // the last level's runs of kChunk values at distance `stride` cover every face
+    // the faces' sum trees: a face holds at most ceil(n / kChunk) chunks, so passes of
+    // strides 1, kChunk, ... below that bound leave every face's total at its first chunk
     const int64_t face_chunks_max = (n + dev::kChunk - 1) / dev::kChunk;
     int64_t stride = 1;
-    for (; stride * dev::kChunk < face_chunks_max; stride *= dev::kChunk)
-        hipLaunchKernelGGL(dev::chunk_tree_level, dim3((unsigned)((mc + 255) / 256)), dim3(256), 0, stream, n_faces, mc,
-                           stride, cbase, cface, part);
-    hipLaunchKernelGGL(dev::face_totals, dim3((unsigned)((n_faces + 255) / 256)), dim3(256), 0, stream, n_faces, mc,
-                       stride, cbase, part, acc);
+    for (int j = 0; j < dev::kTreeLevels && stride < face_chunks_max; ++j, stride *= dev::kChunk)
+        hipLaunchKernelGGL(dev::chunk_tree_level, dim3((unsigned)std::min<int64_t>((mc + 255) / 256, 2048)), dim3(256), 0,
+                           stream, n_faces, mc, stride, need + j, cbase, cface, part);
     if (!have_tables) {
         corners = (const uint32_t*)(w + L.corners);
         corner_begin = (const uint32_t*)(w + L.vbeg);
@@ -440,7 +444,7 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(dev::vertex_sums, dim3((unsigned)((n_vertices + 255) / 256)), dim3(256), 0, stream, n_vertices,
-                       3 * n_faces, corners, corner_begin, corner_end, acc, grad_vertices);
+                       3 * n_faces, mc, corners, corner_begin, corner_end, cbase, part, grad_vertices);
     return hipGetLastError();
 }
 
