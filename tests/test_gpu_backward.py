@@ -366,3 +366,40 @@ def test_backward_with_out_of_range_corner_table_entries(gpu):
     torch.cuda.synchronize()
     assert torch.equal(gp, gp0)
     assert (gv == 0).all() and (gv0 != 0).any()
+
+
+def test_backward_faces_of_one_chunk_bit_exact(gpu, oracle):
+    """Every face under at most 32 queries (a few points just off each face): no face has
+    a second chunk, so every sum-tree pass is skipped and each face's total is its one
+    chunk sum; gradients equal the oracle's bit for bit."""
+    import ctypes
+
+    from cubvh_amd import _lib
+
+    v, f = S.jittered_sphere(3, seed=6)
+    tri = v[f].astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    rng = np.random.default_rng(61)
+    w = rng.dirichlet([4.0, 4.0, 4.0], size=(3, len(f)))
+    p = np.concatenate([np.einsum("fk,fkd->fd", w[i], tri) + 0.004 * (i + 1) * nrm for i in range(3)])
+    p = p.astype(np.float32)[rng.permutation(3 * len(f))]
+    n = len(p)
+    bvh = cubvh_amd.cuBVH(v, f)
+    P = torch.from_numpy(p).to(gpu)
+    Vt = torch.from_numpy(v).to(gpu)
+    Ft = torch.from_numpy(f).to(gpu)
+    d, fid, uvw = bvh.unsigned_distance(P, return_uvw=True)
+    assert np.bincount(fid.cpu().numpy(), minlength=len(f)).max() <= 32
+    g = torch.linspace(0.5, 1.5, n, device=gpu)
+    gp = torch.empty_like(P)
+    gv = torch.zeros_like(Vt)
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    _lib.check(lib.cubvh_udf_backward(gpu.index, n, g.data_ptr(), P.data_ptr(), fid.data_ptr(), uvw.data_ptr(),
+                                      Vt.data_ptr(), len(v), Ft.data_ptr(), len(f), gp.data_ptr(), gv.data_ptr(), s))
+    torch.cuda.synchronize()
+    ogp, ogv = oracle.udf_backward(v, f, p, g.cpu().numpy(), fid.cpu().numpy(), uvw.cpu().numpy())
+    np.testing.assert_array_equal(gp.cpu().numpy(), ogp)
+    np.testing.assert_array_equal(gv.cpu().numpy(), ogv)
+    assert (ogv != 0).any()
