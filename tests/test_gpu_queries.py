@@ -501,3 +501,58 @@ def test_points_on_the_surface_bit_exact(gpu, oracle, ico4):
             np.testing.assert_array_equal(_np(fi), of)
             np.testing.assert_array_equal(_np(uvw), ou)
     assert (oracle.udf_brute(v, f, p)[0] == 0).sum() > 500
+
+
+def _random_soup(seed):
+    """A random triangle soup: 200 to 5,000 faces of random sizes (some slivers) at a
+    random scale and offset, vertices shared between some faces."""
+    rng = np.random.default_rng(seed)
+    nf = int(rng.integers(200, 5000))
+    scale = float(10.0 ** rng.uniform(-3, 3))
+    centre = rng.normal(0.0, 2.0, size=3) * scale
+    c = rng.uniform(-1, 1, size=(nf, 1, 3))
+    e = rng.normal(0, 1, size=(nf, 3, 3)) * rng.uniform(0.01, 0.3, size=(nf, 1, 1))
+    e[: nf // 10, 2] = 0.5 * (e[: nf // 10, 0] + e[: nf // 10, 1])  # slivers
+    v = ((c + e).reshape(-1, 3) * scale + centre).astype(np.float32)
+    f = np.arange(3 * nf, dtype=np.int32).reshape(-1, 3)
+    share = rng.random(nf) < 0.3  # a corner moved onto the previous face's vertex
+    f[1:][share[1:], 0] = f[:-1][share[1:], 1]
+    return v, f, scale, centre
+
+
+@pytest.mark.parametrize("seed", [101, 102, 103, 104, 105])
+def test_random_soups_bit_exact(gpu, oracle, seed):
+    """Random triangle soups over six orders of magnitude of scale: distances, faces and
+    uvw, both signed-distance modes and rays equal the oracle's brute force bit for bit,
+    on the ordered packet path (3,000 queries) and the per-lane path (300); the queries
+    mix uniform points with points just off the surface."""
+    v, f, scale, centre = _random_soup(seed)
+    rng = np.random.default_rng(seed + 1000)
+    t = v[f[rng.integers(0, len(f), 1500)]].astype(np.float64)
+    w = rng.dirichlet([1.0, 1.0, 1.0], size=1500)
+    near = np.einsum("nk,nkd->nd", w, t) + rng.normal(0, 1e-3 * scale, size=(1500, 3))
+    uni = rng.uniform(-1.5, 1.5, size=(1500, 3)) * scale + centre
+    p = np.concatenate([near, uni]).astype(np.float32)[rng.permutation(3000)]
+    bvh = cubvh_amd.cuBVH(v, f)
+    for q in (p, p[:300]):
+        Q = torch.from_numpy(q).to(gpu)
+        d, fi, uvw = bvh.unsigned_distance(Q, return_uvw=True)
+        od, of, ou = oracle.udf_brute(v, f, q)
+        np.testing.assert_array_equal(_np(d), od)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(uvw), ou)
+        for mode, m in (("watertight", 0), ("raystab", 1)):
+            d, fi, _ = bvh.signed_distance(Q, mode=mode)
+            od, of, _ = oracle.sdf_brute(v, f, q, m, return_uvw=False)
+            np.testing.assert_array_equal(_np(d), od)
+            np.testing.assert_array_equal(_np(fi), of)
+    ro = (rng.uniform(-2, 2, size=(3000, 3)) * scale + centre).astype(np.float32)
+    aim = v[f[rng.integers(0, len(f), 3000), 0]]
+    rd = aim - ro
+    rd = (rd / np.linalg.norm(rd, axis=1, keepdims=True)).astype(np.float32)
+    for n in (3000, 300):
+        pos, fi, tt = bvh.ray_trace(torch.from_numpy(ro[:n]).to(gpu), torch.from_numpy(rd[:n]).to(gpu))
+        opos, of, ot = oracle.ray_brute(v, f, ro[:n], rd[:n])
+        np.testing.assert_array_equal(_np(tt), ot)
+        np.testing.assert_array_equal(_np(fi), of)
+        np.testing.assert_array_equal(_np(pos), opos)
