@@ -525,7 +525,8 @@ def test_random_soups_bit_exact(gpu, oracle, seed):
     """Random triangle soups over six orders of magnitude of scale: distances, faces and
     uvw, both signed-distance modes and rays equal the oracle's brute force bit for bit,
     on the ordered packet path (3,000 queries) and the per-lane path (300); the queries
-    mix uniform points with points just off the surface."""
+    mix uniform points with points just off the surface.  The three GPU-built trees give
+    the host tree's answers."""
     v, f, scale, centre = _random_soup(seed)
     rng = np.random.default_rng(seed + 1000)
     t = v[f[rng.integers(0, len(f), 1500)]].astype(np.float64)
@@ -556,3 +557,14 @@ def test_random_soups_bit_exact(gpu, oracle, seed):
         np.testing.assert_array_equal(_np(tt), ot)
         np.testing.assert_array_equal(_np(fi), of)
         np.testing.assert_array_equal(_np(pos), opos)
+    # the GPU-built trees (object median, SAH, linear BVH) give the host tree's answers
+    Q = torch.from_numpy(p).to(gpu)
+    ref = [_np(x) for x in bvh.unsigned_distance(Q, return_uvw=True)]
+    R = [torch.from_numpy(x).to(gpu) for x in (ro, rd)]
+    rref = [_np(x) for x in bvh.ray_trace(*R)]
+    for build in ("gpu", "gpu_sah", "gpu_lbvh"):
+        g = cubvh_amd.cuBVH(v, f, build=build)
+        for x, y in zip(g.unsigned_distance(Q, return_uvw=True), ref):
+            np.testing.assert_array_equal(_np(x), y)
+        for x, y in zip(g.ray_trace(*R), rref):
+            np.testing.assert_array_equal(_np(x), y)
