@@ -142,3 +142,42 @@ def test_forest_nan_and_inf_queries_match_per_mesh(gpu):
         np.testing.assert_array_equal(d.cpu().numpy(), rd)
         np.testing.assert_array_equal(f.cpu().numpy(), rf)
         np.testing.assert_array_equal(u.cpu().numpy(), ru)
+
+
+def test_forest_of_random_soups_matches_per_mesh(gpu):
+    """A batch of random triangle soups at scales from 1e-3 to 1e3 (one mesh without
+    queries): distances, faces and uvw equal the per-mesh calls for three launch shapes,
+    and the batched backward's point and vertex gradients equal the per-mesh ones."""
+    from test_gpu_queries import _random_soup
+
+    bvhs, pts, counts = [], [], []
+    rng = np.random.default_rng(71)
+    for k, seed in enumerate((111, 112, 113, 114, 115, 116)):
+        v, f, scale, centre = _random_soup(seed)
+        bvhs.append(cubvh_amd.cuBVH(v, f))
+        nq = 0 if k == 2 else int(rng.integers(1, 6000))
+        pts.append((rng.uniform(-1.5, 1.5, size=(nq, 3)) * scale + centre).astype(np.float32))
+        counts.append(nq)
+    pts = np.concatenate(pts)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    forest = cubvh_amd.cuBVHForest(bvhs)
+    rd, rf, ru = _per_mesh(bvhs, pts, offsets, gpu)
+    for tuning in [(12, 1, 8), (15, 0, 8, 1, 64), (0, 0, 8, 2)]:
+        d, f, u = forest.unsigned_distance(torch.from_numpy(pts).to(gpu), torch.from_numpy(offsets), True, tuning)
+        assert torch.equal(d, rd) and torch.equal(f, rf) and torch.equal(u, ru)
+    P = torch.from_numpy(pts).to(gpu).requires_grad_(True)
+    vcat, _ = forest.concatenated_mesh(gpu)
+    V = vcat.clone().requires_grad_(True)
+    d, _, _ = cubvh_amd.udf_batched(forest, P, torch.from_numpy(offsets), V)
+    (d * d).sum().backward()
+    gps, gvs = [], []
+    for m, b in enumerate(bvhs):
+        p = torch.from_numpy(pts[offsets[m]:offsets[m + 1]]).to(gpu).requires_grad_(True)
+        v = torch.from_numpy(b._vertices).to(gpu).requires_grad_(True)
+        if p.shape[0]:
+            dd, _, _ = cubvh_amd.udf(b, p, v)
+            (dd * dd).sum().backward()
+            gps.append(p.grad)
+        gvs.append(v.grad if v.grad is not None else torch.zeros_like(v))
+    assert torch.equal(P.grad, torch.cat(gps))
+    assert torch.equal(V.grad, torch.cat(gvs))
