@@ -48,6 +48,7 @@
 #include <stdint.h>
 
 #include "query.hpp"
+#include "radix.hpp"
 
 namespace cubvh_amd {
 namespace dev {
@@ -256,8 +257,7 @@ int key_bits(int64_t top) {
 
 size_t sort_temp_bytes(int64_t m, int bits) {
     size_t tmp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)m, 0, bits);
+    radix_sort_pairs_u32(nullptr, tmp, nullptr, nullptr, nullptr, nullptr, m, bits, 0);
     return tmp;
 }
 
@@ -362,8 +362,7 @@ hipError_t launch_vertex_corners(const int32_t* faces, int64_t n_faces, int64_t 
     hipLaunchKernelGGL(dev::corner_keys, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, nc, faces, n_vertices,
                        c0, cv0);
     size_t tb = L.tmp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(w + L.tmp), tb, c0, c1, cv0, corners, (int)nc, 0,
-                                                      key_bits(n_vertices), stream);
+    hipError_t e = radix_sort_pairs_u32((void*)(w + L.tmp), tb, c0, c1, cv0, corners, nc, key_bits(n_vertices), stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::run_bounds, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, nc, c1, n_vertices,
                        begin, end);
@@ -399,7 +398,7 @@ hipError_t launch_udf_backward(int64_t n, const float* grad_dist, const float* p
     hipLaunchKernelGGL(dev::query_terms, dim3(gq), dim3(256), 0, stream, n, grad_dist, positions, face_id, uvw,
                        vertices, faces, n_vertices, n_faces, gn, k0, v0);
     size_t tb = L.tmp_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)n, 0, key_bits(n_faces), stream);
+    e = radix_sort_pairs_u32(tmp, tb, k0, k1, v0, v1, n, key_bits(n_faces), stream);
     if (e != hipSuccess) return e;
     uint32_t* fbeg = (uint32_t*)(w + L.fbeg);
     uint32_t* fend = (uint32_t*)(w + L.fend);

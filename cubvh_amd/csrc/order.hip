@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "query.hpp"
+#include "radix.hpp"
 
 namespace cubvh_amd {
 namespace dev {
@@ -171,8 +172,7 @@ hipError_t launch_bounds(int64_t n, const float* positions, int32_t* bounds, hip
 // Bytes of device workspace morton_order needs for n queries.
 size_t morton_workspace_bytes(int64_t n) {
     size_t tmp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
+    radix_sort_pairs_u32(nullptr, tmp, nullptr, nullptr, nullptr, nullptr, n, 30, 0);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     return al(64) + 3 * al(4 * (size_t)n) + al(tmp);
 }
@@ -192,13 +192,13 @@ hipError_t ray_order(int64_t n, const float* rays_o, const float* rays_d, void* 
     uint32_t* v0 = (uint32_t*)w;
     w += al(4 * (size_t)n);
     size_t tmp = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    hipError_t e = radix_sort_pairs_u32(nullptr, tmp, k0, k1, v0, order_out, n, 30, s);
     if (e != hipSuccess) return e;
     e = launch_bounds(n, rays_o, bounds, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::ray_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, rays_o, rays_d, bounds,
                        k0, v0);
-    e = hipcub::DeviceRadixSort::SortPairs((void*)w, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    e = radix_sort_pairs_u32((void*)w, tmp, k0, k1, v0, order_out, n, 30, s);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -245,13 +245,13 @@ hipError_t morton_order(int64_t n, const float* positions, void* ws, uint32_t* o
     uint32_t* v0 = (uint32_t*)w;
     w += al(4 * (size_t)n);
     size_t tmp = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    hipError_t e = radix_sort_pairs_u32(nullptr, tmp, k0, k1, v0, order_out, n, 30, s);
     if (e != hipSuccess) return e;
     e = launch_bounds(n, positions, bounds, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dev::morton_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, positions, bounds, k0,
                        v0);
-    e = hipcub::DeviceRadixSort::SortPairs((void*)w, tmp, k0, k1, v0, order_out, (int)n, 0, 30, s);
+    e = radix_sort_pairs_u32((void*)w, tmp, k0, k1, v0, order_out, n, 30, s);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
